@@ -1,0 +1,282 @@
+/*
+ * ub_solve.c -- per-pixel constrained solve of the oracle (test infrastructure).
+ *
+ * Outer loop: the reference's augmented-Lagrangian iteration,
+ * unblending/src/unblending.cpp:155-309, with its constants (:220-224) and the
+ * default initial solution (:116-153, :185-208).
+ *
+ * Inner loop: the reference calls NLopt LD_LBFGS (unblending.cpp:249).  NLopt
+ * is not available (un-vendored submodule), so this file defines the inner
+ * minimiser the whole project uses -- a projected limited-memory BFGS:
+ *
+ *   1. free set: i is free unless lb_i == ub_i, or x_i sits on a bound with the
+ *      gradient pointing outward; q = g restricted to the free set.
+ *      Stop when max|q_i| == 0.
+ *   2. direction: the two-loop recursion over the stored (s, y) pairs
+ *      (newest to oldest, then back), initial scaling s.y / y.y of the newest
+ *      pair; d = -H q on the free set, 0 elsewhere.  If g.d is not negative the
+ *      history is dropped and d = -q.
+ *   3. step: t = 1, or min(1, 1/|d|) with an empty history.  Trial points
+ *      x(t) = clamp(x + t d, lb, ub); accept when f(x(t)) <= f + 1e-4 g.(x(t) - x).
+ *      Otherwise t <- the minimiser of the quadratic through f, g.(x(t)-x) and
+ *      f(x(t)), clamped to [0.1 t, 0.5 t] (0.5 t when g.(x(t)-x) >= 0).  At most
+ *      max_ls trials; a failed search ends the inner solve at the last
+ *      accepted point (the monotone minimiser's best point).
+ *   4. update: s = x+ - x, y = g+ - g; the pair is kept when s.y > 2^-52 y.y
+ *      (memory pairs, oldest dropped), otherwise the whole history is dropped.
+ *   5. stop (NLopt's relative tests, nlopt util/stop.c semantics):
+ *      |f+ - f| < ftol_rel (|f+| + |f|)/2 or f+ == f; or for every i
+ *      |x+_i - x_i| < xtol_rel (|x+_i| + |x_i|)/2 or x+_i == x_i; or the
+ *      evaluation count reaches max_evals; or f or g stops being finite.
+ *
+ * Every reduction is a left-to-right sum in index order.
+ */
+#include "ub_oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+void orc_default_settings(orc_settings* s)
+{
+    s->memory = 4;
+    s->max_evals = 1000;   /* unblending.cpp:249 */
+    s->max_count = 20;     /* unblending.cpp:246 */
+    s->max_ls = 20;
+    s->ftol_rel = 5e-3;    /* unblending.cpp:222 local_epsilon */
+    s->xtol_rel = 5e-3;
+    s->epsilon = 5e-3;     /* unblending.cpp:221 */
+    s->gamma = 0.25;       /* :220 */
+    s->beta = 10.0;        /* :223 */
+    s->initial_rho = 100.0;/* :224 */
+    s->alpha_fixed = 0;    /* MATTE_ALPHA_FORCE_FIXED off, unblending.cpp:19 */
+}
+
+static double clampd(double v, double lo, double hi)
+{
+    v = (v < lo) ? lo : v;
+    return (v > hi) ? hi : v;
+}
+
+static double dot(const double* a, const double* b, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+static int relstop(double vold, double vnew, double reltol)
+{
+    return fabs(vnew - vold) < reltol * (fabs(vnew) + fabs(vold)) * 0.5 || vnew == vold;
+}
+
+int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn, void* data,
+                 const orc_settings* st, double* f_out, int32_t* evals_out, int32_t* iterations_out)
+{
+    const int M = st->memory < 1 ? 1 : (st->memory > ORC_MAX_MEMORY ? ORC_MAX_MEMORY : st->memory);
+    double g[ORC_MAX_VARS], d[ORC_MAX_VARS], q[ORC_MAX_VARS], xt[ORC_MAX_VARS], gt[ORC_MAX_VARS];
+    double S[ORC_MAX_MEMORY][ORC_MAX_VARS], Yh[ORC_MAX_MEMORY][ORC_MAX_VARS];
+    double rh[ORC_MAX_MEMORY], a[ORC_MAX_MEMORY];
+    int freev[ORC_MAX_VARS];
+    int count = 0, head = 0; /* ring: slot (head + k) % M holds the k-th oldest pair */
+    double gamma = 1.0;
+
+    for (int i = 0; i < n; ++i) x[i] = clampd(x[i], lb[i], ub[i]);
+    double f = fn(x, g, data);
+    int32_t nev = 1, iters = 0;
+
+    for (;;) {
+        int finite = isfinite(f);
+        for (int i = 0; i < n; ++i) finite = finite && isfinite(g[i]);
+        if (!finite) break;
+
+        double pgmax = 0.0;
+        for (int i = 0; i < n; ++i) {
+            freev[i] = lb[i] < ub[i] && !(x[i] <= lb[i] && g[i] > 0.0) && !(x[i] >= ub[i] && g[i] < 0.0);
+            q[i] = freev[i] ? g[i] : 0.0;
+            const double m = fabs(q[i]);
+            pgmax = (m > pgmax) ? m : pgmax;
+        }
+        if (pgmax == 0.0) break;
+        ++iters;
+
+        /* two-loop recursion */
+        for (int k = count - 1; k >= 0; --k) {
+            const int j = (head + k) % M;
+            a[k] = rh[j] * dot(S[j], q, n);
+            for (int i = 0; i < n; ++i) q[i] = q[i] - a[k] * Yh[j][i];
+        }
+        for (int i = 0; i < n; ++i) d[i] = gamma * q[i];
+        for (int k = 0; k < count; ++k) {
+            const int j = (head + k) % M;
+            const double b = rh[j] * dot(Yh[j], d, n);
+            for (int i = 0; i < n; ++i) d[i] = d[i] + (a[k] - b) * S[j][i];
+        }
+        for (int i = 0; i < n; ++i) d[i] = freev[i] ? -d[i] : 0.0;
+        double gd = dot(g, d, n);
+        if (!(gd < 0.0)) {
+            count = 0;
+            head = 0;
+            gamma = 1.0;
+            for (int i = 0; i < n; ++i) d[i] = freev[i] ? -g[i] : 0.0;
+        }
+
+        double t = 1.0;
+        if (count == 0) {
+            const double inv = 1.0 / sqrt(dot(d, d, n));
+            t = (inv < 1.0) ? inv : 1.0;
+        }
+
+        int accepted = 0;
+        double ft = 0.0;
+        for (int ls = 0;; ++ls) {
+            for (int i = 0; i < n; ++i) xt[i] = clampd(x[i] + t * d[i], lb[i], ub[i]);
+            ft = fn(xt, gt, data);
+            ++nev;
+            double dphi = 0.0;
+            for (int i = 0; i < n; ++i) dphi += g[i] * (xt[i] - x[i]);
+            if (isfinite(ft) && ft <= f + 1e-4 * dphi) {
+                accepted = 1;
+                break;
+            }
+            if (nev >= st->max_evals || ls + 1 >= st->max_ls) break;
+            double tn;
+            if (dphi < 0.0) {
+                const double denom = 2.0 * (ft - f - dphi);
+                tn = (-dphi * t) / denom;
+            } else {
+                tn = 0.5 * t;
+            }
+            const double lo = 0.1 * t, hi = 0.5 * t;
+            tn = (tn >= lo) ? tn : lo; /* also replaces NaN */
+            tn = (tn > hi) ? hi : tn;
+            t = tn;
+        }
+        if (!accepted) break;
+
+        /* update */
+        double sy = 0.0, yy = 0.0;
+        int xstop = 1;
+        const int slot = (count < M) ? (head + count) % M : head;
+        for (int i = 0; i < n; ++i) {
+            const double s = xt[i] - x[i];
+            const double y = gt[i] - g[i];
+            sy += s * y;
+            yy += y * y;
+            xstop = xstop && (fabs(s) < st->xtol_rel * (fabs(xt[i]) + fabs(x[i])) * 0.5 || xt[i] == x[i]);
+            S[slot][i] = s;
+            Yh[slot][i] = y;
+            x[i] = xt[i];
+            g[i] = gt[i];
+        }
+        const double fold = f;
+        f = ft;
+        if (sy > 2.220446049250313e-16 * yy) {
+            rh[slot] = 1.0 / sy;
+            gamma = sy / yy;
+            if (count < M) ++count;
+            else head = (head + 1) % M;
+        } else {
+            count = 0;
+            head = 0;
+            gamma = 1.0;
+        }
+        if (nev >= st->max_evals) break;
+        if (relstop(fold, f, st->ftol_rel)) break;
+        if (xstop) break;
+    }
+    if (f_out) *f_out = f;
+    if (evals_out) *evals_out = nev;
+    if (iterations_out) *iterations_out = iters;
+    return 0;
+}
+
+typedef struct {
+    int L;
+    const orc_layer* layers;
+    const double* target;
+    const double* target_alphas;
+    double lambda[ORC_MAX_CONSTRAINTS];
+    double rho;
+} pixel_problem;
+
+static double pixel_fn(const double* x, double* grad, void* data)
+{
+    const pixel_problem* p = (const pixel_problem*)data;
+    double C[ORC_MAX_CONSTRAINTS];
+    return orc_objective(x, p->L, p->layers, p->target, p->target_alphas, p->lambda, p->rho, grad, C);
+}
+
+static double norm(const double* v, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += v[i] * v[i];
+    return sqrt(s);
+}
+
+/* unblending.cpp:155-309 */
+int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int opaque,
+                    const double* init_colors, const double* target_alphas,
+                    const double* bg_color, const orc_settings* st, double* x_out,
+                    orc_counters* counters)
+{
+    const int N = 4 * L;
+    const int refine = target_alphas != NULL;
+    double lb[ORC_MAX_VARS], ub[ORC_MAX_VARS], x[ORC_MAX_VARS], xn[ORC_MAX_VARS];
+    for (int i = 0; i < N; ++i) {
+        lb[i] = 0.0;
+        ub[i] = 1.0;
+    }
+    if (refine && st->alpha_fixed)
+        for (int i = 0; i < L; ++i) lb[i] = ub[i] = target_alphas[i];
+
+    /* find_initial_solution, :145-149; refinement initial values, :187-191 */
+    for (int i = 0; i < L; ++i) {
+        x[i] = refine ? target_alphas[i] : 0.50;
+        for (int c = 0; c < 3; ++c)
+            x[L + 3 * i + c] = refine ? init_colors[3 * i + c] : layers[i].mean[c];
+    }
+    if (opaque) { /* :194-198 */
+        lb[0] = 1.0;
+        if (refine && st->alpha_fixed) ub[0] = 1.0;
+        x[0] = 1.0;
+    }
+    if (bg_color) { /* :201-208 */
+        for (int c = 0; c < 3; ++c) lb[L + c] = ub[L + c] = x[L + c] = bg_color[c];
+    }
+
+    pixel_problem p;
+    p.L = L;
+    p.layers = layers;
+    p.target = target;
+    p.target_alphas = target_alphas;
+    const int nc = refine ? 3 + L : 4;
+    for (int j = 0; j < nc; ++j) p.lambda[j] = 0.0;
+    p.rho = st->initial_rho;
+
+    int count = 0;
+    for (;;) {
+        memcpy(xn, x, sizeof(double) * (size_t)N);
+        int32_t ev = 0, it = 0;
+        orc_minimize(xn, lb, ub, N, pixel_fn, &p, st, NULL, &ev, &it);
+        if (counters) {
+            counters->evals += ev;
+            counters->iterations += it;
+            counters->outer += 1;
+        }
+        double g[ORC_MAX_CONSTRAINTS], gn[ORC_MAX_CONSTRAINTS], dx[ORC_MAX_VARS];
+        orc_constraints(x, L, layers, target, target_alphas, g);
+        orc_constraints(xn, L, layers, target, target_alphas, gn);
+        for (int j = 0; j < nc; ++j) p.lambda[j] = p.lambda[j] - p.rho * gn[j];
+        const double ngn = norm(gn, nc);
+        if (ngn > st->gamma * norm(g, nc)) p.rho = p.rho * st->beta;
+        for (int i = 0; i < N; ++i) dx[i] = xn[i] - x[i];
+        const int unchanged = norm(dx, N) < st->epsilon;
+        const int satisfied = ngn < st->epsilon;
+        memcpy(x, xn, sizeof(double) * (size_t)N);
+        if ((unchanged && satisfied) || count > st->max_count) break;
+        ++count;
+    }
+    memcpy(x_out, x, sizeof(double) * (size_t)N);
+    if (counters) counters->pixels += 1;
+    return 0;
+}

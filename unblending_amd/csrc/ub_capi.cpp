@@ -1,0 +1,418 @@
+// ub_capi.cpp -- the exported C interface of the HIP backend.
+//
+// Argument checks follow the reference's C ABI and packed-buffer API in the
+// same order (unblending/sdk/unblending_c.cpp:23-92 and
+// unblending/src/unblending.cpp:508-534, 590-614), with the same status codes
+// (unblending_c.cpp:87-91).  The solve itself runs on the current HIP device
+// through ub_pipeline.cpp; there is no CPU solver in this library.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ub_host.hpp"
+
+namespace ubhip {
+
+thread_local std::string g_last_error;
+
+namespace {
+
+enum class Backend { automatic, cpu, cuda, hip };
+
+// unblending_c.cpp:12-21, plus the HIP value.
+Backend select_backend(int32_t backend)
+{
+    switch (backend) {
+    case UNBLENDING_BACKEND_AUTO: return Backend::automatic;
+    case UNBLENDING_BACKEND_CPU: return Backend::cpu;
+    case UNBLENDING_BACKEND_CUDA: return Backend::cuda;
+    case UNBLENDING_BACKEND_HIP: return Backend::hip;
+    default: throw std::invalid_argument("Unknown backend");
+    }
+}
+
+// Eigen's isApprox for P against P^T: |P - P^T|^2 <= prec^2 min(|P|^2, |P^T|^2).
+bool symmetric_approx(const double* P)
+{
+    double diff = 0.0, norm = 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const double d = P[3 * r + c] - P[3 * c + r];
+            diff += d * d;
+            norm += P[3 * r + c] * P[3 * r + c];
+        }
+    return diff <= 1e-12 * 1e-12 * norm;
+}
+
+// Cholesky of the lower triangle; fails on a non-positive (or NaN) pivot as
+// Eigen::LLT reports (unblending_c.cpp:42).
+bool cholesky_ok(const double* P)
+{
+    double Lm[3][3] = {{0}};
+    for (int k = 0; k < 3; ++k) {
+        double x = P[3 * k + k];
+        for (int j = 0; j < k; ++j) x -= Lm[k][j] * Lm[k][j];
+        if (!(x > 0.0)) return false;
+        Lm[k][k] = std::sqrt(x);
+        for (int i = k + 1; i < 3; ++i) {
+            double y = P[3 * i + k];
+            for (int j = 0; j < k; ++j) y -= Lm[i][j] * Lm[k][j];
+            Lm[i][k] = y / Lm[k][k];
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+// unblending_c.cpp:23-48
+LayerSet make_layer_set(const unblending_layer* layers, size_t count)
+{
+    LayerSet set;
+    set.host.assign(layers, layers + count);
+    set.all_plus = true;
+    set.all_so = true;
+    for (size_t i = 0; i < count; ++i) {
+        const unblending_layer& l = layers[i];
+        if (l.blend_mode < UNBLENDING_NORMAL || l.blend_mode > UNBLENDING_LINEAR_DODGE)
+            throw std::invalid_argument("Unknown blend mode");
+        bool finite = true;
+        for (double v : l.mean) finite = finite && std::isfinite(v);
+        for (double v : l.precision) finite = finite && std::isfinite(v);
+        if (!finite || !symmetric_approx(l.precision) || !cholesky_ok(l.precision))
+            throw std::invalid_argument("Invalid Gaussian color model");
+        set.all_plus = set.all_plus && l.comp_x == 2 && l.comp_y == 1 && l.comp_z == 1;
+        set.all_so = set.all_so && l.comp_x == 1 && l.comp_y == 1 && l.comp_z == 1;
+    }
+    return set;
+}
+
+// unblending.cpp:508-534: normalises the stride and returns the packed size.
+size_t validate_pixels(unblending_rgb_view& image, size_t layer_count)
+{
+    constexpr auto max_int = std::numeric_limits<int>::max();
+    if (!image.data || image.width <= 0 || image.height <= 0 || image.width > max_int / image.height)
+        throw std::invalid_argument("Invalid RGB image dimensions or data");
+    const size_t max_doubles = std::vector<double>().max_size();
+    if (size_t(image.width) > max_doubles / 3) throw std::length_error("RGB row is too large");
+    const size_t row_size = 3 * size_t(image.width);
+    if (!image.row_stride) image.row_stride = row_size;
+    if (image.row_stride < row_size || image.size < row_size ||
+        size_t(image.height - 1) > (image.size - row_size) / image.row_stride)
+        throw std::invalid_argument("RGB buffer is too small or has an invalid row stride");
+    if (layer_count == 0) throw std::invalid_argument("Each layer requires a color model");
+    const size_t pixels = size_t(image.width) * size_t(image.height);
+    if (layer_count > size_t(max_int) / 4 || pixels > max_doubles / 4 / layer_count)
+        throw std::length_error("RGBA layer stack is too large");
+    return 4 * pixels * layer_count;
+}
+
+namespace {
+
+struct FirstPass {
+    const double* data;
+    size_t size;
+};
+
+// unblending_c.cpp:50-92 with the HIP dispatch of gpu_dispatch.cpp:52-123.
+int32_t solve(const unblending_rgb_view* image, const unblending_layer* layers, size_t layer_count,
+              const unblending_options* options, const FirstPass* first,
+              const unblending_hip_params* params, double* output, size_t output_capacity,
+              unblending_hip_stats* stats)
+{
+    g_last_error.clear();
+    try {
+        constexpr auto max_int = std::numeric_limits<int>::max();
+        if (!image || !layers || !layer_count || layer_count > size_t(max_int) / 4 || image->width <= 0 ||
+            image->height <= 0 || image->width > max_int / image->height)
+            return UNBLENDING_INVALID_ARGUMENT;
+        const size_t pixels = size_t(image->width) * size_t(image->height);
+        if (pixels > std::vector<double>().max_size() / 4 / layer_count) return UNBLENDING_INVALID_ARGUMENT;
+        const size_t required = 4 * pixels * layer_count;
+        if (output_capacity < required) return UNBLENDING_BUFFER_TOO_SMALL;
+        if (!output) return UNBLENDING_INVALID_ARGUMENT;
+
+        const unblending_options defaults = UNBLENDING_OPTIONS_INIT;
+        if (!options) options = &defaults;
+        if (options->struct_size < sizeof(unblending_options) ||
+            (options->opaque_background != 0 && options->opaque_background != 1) ||
+            (options->smooth_background != 0 && options->smooth_background != 1) ||
+            options->target_concurrency < 0)
+            return UNBLENDING_INVALID_ARGUMENT;
+        unblending_hip_params p = UNBLENDING_HIP_PARAMS_INIT;
+        if (params) {
+            if (params->struct_size < sizeof(unblending_hip_params)) return UNBLENDING_INVALID_ARGUMENT;
+            p = *params;
+        }
+
+        const Backend backend = select_backend(options->backend);
+        const LayerSet set = make_layer_set(layers, layer_count);
+        const bool opaque = options->opaque_background != 0;
+        const bool smooth = options->smooth_background != 0;
+
+        unblending_rgb_view rgb = *image;
+        const size_t size = validate_pixels(rgb, layer_count);
+        if (first) {
+            // unblending.cpp:597-604
+            if (!first->data || first->size != size)
+                throw std::invalid_argument("Refinement requires one packed RGBA image per layer");
+            if (smooth && !opaque) throw std::invalid_argument("Smoothing requires an opaque background");
+            if (!set.all_plus && !set.all_so)
+                throw std::invalid_argument("Refinement requires an all-Plus or all-SourceOver stack");
+        }
+        if (backend == Backend::cpu)
+            throw std::invalid_argument("The CPU backend is not part of the HIP library");
+        if (backend == Backend::cuda) throw ExecutionError("This library is built without CUDA");
+
+        host_solve(rgb, set, first ? first->data : nullptr, opaque, smooth && first != nullptr, p, output, stats);
+        return UNBLENDING_OK;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of memory";
+        return UNBLENDING_OUT_OF_MEMORY;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return UNBLENDING_INVALID_ARGUMENT;
+    } catch (const std::length_error& e) {
+        g_last_error = e.what();
+        return UNBLENDING_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    } catch (...) {
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
+// Checks shared by the device-resident calls.
+int32_t device_call(const double* rgb, int32_t width, int32_t height, const unblending_layer* layers,
+                    size_t layer_count, const double* first, const double* alpha_planes,
+                    const unblending_options* options, const unblending_hip_params* params,
+                    double* output, double* alpha_out, void* workspace, size_t workspace_size,
+                    void* stream, unblending_hip_stats* stats)
+{
+    g_last_error.clear();
+    try {
+        constexpr auto max_int = std::numeric_limits<int>::max();
+        if (!rgb || !layers || !layer_count || layer_count > size_t(max_int) / 4 || width <= 0 ||
+            height <= 0 || width > max_int / height || !output)
+            return UNBLENDING_INVALID_ARGUMENT;
+        const unblending_options defaults = UNBLENDING_OPTIONS_INIT;
+        if (!options) options = &defaults;
+        if (options->struct_size < sizeof(unblending_options) ||
+            (options->opaque_background != 0 && options->opaque_background != 1) ||
+            (options->smooth_background != 0 && options->smooth_background != 1) ||
+            options->target_concurrency < 0)
+            return UNBLENDING_INVALID_ARGUMENT;
+        unblending_hip_params p = UNBLENDING_HIP_PARAMS_INIT;
+        if (params) {
+            if (params->struct_size < sizeof(unblending_hip_params)) return UNBLENDING_INVALID_ARGUMENT;
+            p = *params;
+        }
+        const Backend backend = select_backend(options->backend);
+        if (backend == Backend::cpu) return UNBLENDING_INVALID_ARGUMENT;
+        if (backend == Backend::cuda) return UNBLENDING_DEVICE_ERROR;
+        const LayerSet set = make_layer_set(layers, layer_count);
+        const bool opaque = options->opaque_background != 0;
+        const bool smooth = options->smooth_background != 0;
+        if (first) {
+            if (smooth && !opaque) return UNBLENDING_INVALID_ARGUMENT;
+            if (!set.all_plus && !set.all_so) return UNBLENDING_INVALID_ARGUMENT;
+        }
+        const size_t need = unblending_hip_workspace_size(width, height, layer_count, first != nullptr,
+                                                          smooth && first != nullptr);
+        if (!workspace || workspace_size < need) return UNBLENDING_BUFFER_TOO_SMALL;
+        DeviceJob job;
+        job.w = width;
+        job.h = height;
+        job.set = &set;
+        job.rgb = rgb;
+        job.first = first;
+        job.alpha_in = alpha_planes;
+        job.out = output;
+        job.alpha_out = alpha_out;
+        job.opaque = opaque;
+        job.smooth = smooth && first != nullptr;
+        job.params = p;
+        job.workspace = workspace;
+        job.workspace_size = workspace_size;
+        job.stream = static_cast<hipStream_t>(stream);
+        job.stats = stats;
+        run_device(job);
+        return UNBLENDING_OK;
+    } catch (const std::bad_alloc&) {
+        return UNBLENDING_OUT_OF_MEMORY;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return UNBLENDING_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    } catch (...) {
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
+}  // namespace
+}  // namespace ubhip
+
+using namespace ubhip;
+
+extern "C" {
+
+UNBLENDING_API uint32_t unblending_abi_version(void) { return UNBLENDING_ABI_VERSION; }
+
+UNBLENDING_API int32_t unblending_unmix(const unblending_rgb_view* image, const unblending_layer* layers,
+                                        size_t layer_count, const unblending_options* options,
+                                        double* output, size_t output_capacity)
+{
+    return solve(image, layers, layer_count, options, nullptr, nullptr, output, output_capacity, nullptr);
+}
+
+UNBLENDING_API int32_t unblending_refine(const unblending_rgb_view* image, const unblending_layer* layers,
+                                         size_t layer_count, const double* first_pass, size_t first_pass_size,
+                                         const unblending_options* options, double* output,
+                                         size_t output_capacity)
+{
+    const FirstPass first{first_pass, first_pass_size};
+    return solve(image, layers, layer_count, options, &first, nullptr, output, output_capacity, nullptr);
+}
+
+UNBLENDING_API int32_t unblending_hip_unmix_ex(const unblending_rgb_view* image, const unblending_layer* layers,
+                                               size_t layer_count, const unblending_options* options,
+                                               const unblending_hip_params* params, double* output,
+                                               size_t output_capacity, unblending_hip_stats* stats)
+{
+    return solve(image, layers, layer_count, options, nullptr, params, output, output_capacity, stats);
+}
+
+UNBLENDING_API int32_t unblending_hip_refine_ex(const unblending_rgb_view* image, const unblending_layer* layers,
+                                                size_t layer_count, const double* first_pass,
+                                                size_t first_pass_size, const unblending_options* options,
+                                                const unblending_hip_params* params, double* output,
+                                                size_t output_capacity, unblending_hip_stats* stats)
+{
+    const FirstPass first{first_pass, first_pass_size};
+    return solve(image, layers, layer_count, options, &first, params, output, output_capacity, stats);
+}
+
+UNBLENDING_API size_t unblending_hip_workspace_size(int32_t width, int32_t height, size_t layer_count,
+                                                   int32_t refine, int32_t smooth_background)
+{
+    if (width <= 0 || height <= 0 || layer_count == 0) return 0;
+    return workspace_bytes(size_t(width) * size_t(height), layer_count, refine != 0, smooth_background != 0);
+}
+
+UNBLENDING_API int32_t unblending_hip_unmix_device(const double* rgb, int32_t width, int32_t height,
+                                                   const unblending_layer* layers, size_t layer_count,
+                                                   const unblending_options* options,
+                                                   const unblending_hip_params* params, double* output,
+                                                   double* alpha_planes, void* workspace, size_t workspace_size,
+                                                   void* stream, unblending_hip_stats* stats)
+{
+    return device_call(rgb, width, height, layers, layer_count, nullptr, nullptr, options, params, output,
+                       alpha_planes, workspace, workspace_size, stream, stats);
+}
+
+UNBLENDING_API int32_t unblending_hip_refine_device(const double* rgb, int32_t width, int32_t height,
+                                                    const unblending_layer* layers, size_t layer_count,
+                                                    const double* first_pass, const double* alpha_planes,
+                                                    const unblending_options* options,
+                                                    const unblending_hip_params* params, double* output,
+                                                    void* workspace, size_t workspace_size, void* stream,
+                                                    unblending_hip_stats* stats)
+{
+    if (!first_pass) return UNBLENDING_INVALID_ARGUMENT;
+    return device_call(rgb, width, height, layers, layer_count, first_pass, alpha_planes, options, params,
+                       output, nullptr, workspace, workspace_size, stream, stats);
+}
+
+UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_t height, int32_t channels)
+{
+    if (width <= 0 || height <= 0 || channels <= 0) return 0;
+    return filter_workspace_bytes(size_t(width) * size_t(height), channels);
+}
+
+UNBLENDING_API int32_t unblending_hip_guided_filter_device(const double* rgb, int32_t width, int32_t height,
+                                                           const double* const* p, double* const* q,
+                                                           int32_t channels, int32_t radius, double epsilon,
+                                                           void* workspace, size_t workspace_size, void* stream,
+                                                           unblending_hip_stats* stats)
+{
+    g_last_error.clear();
+    try {
+        if (!rgb || !p || !q || channels <= 0 || width <= 0 || height <= 0 || radius < 0 || !(epsilon > 0.0))
+            return UNBLENDING_INVALID_ARGUMENT;
+        if (!workspace || workspace_size < filter_workspace_bytes(size_t(width) * size_t(height), channels))
+            return UNBLENDING_BUFFER_TOO_SMALL;
+        std::vector<Channel> in(channels);
+        for (int c = 0; c < channels; ++c) in[c] = Channel{p[c], 1};
+        run_filter(rgb, width, height, in.data(), q, channels, radius, epsilon, static_cast<double*>(workspace),
+                   static_cast<hipStream_t>(stream), stats, true);
+        return UNBLENDING_OK;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
+UNBLENDING_API int32_t uhip_solve_planes(int32_t width, int32_t height, const unblending_layer* layers,
+                                         size_t layer_count, const double* rgb_planes, const double* first_planes,
+                                         int32_t opaque, int32_t smooth, double* out_planes)
+{
+    g_last_error.clear();
+    try {
+        if (width <= 0 || height <= 0 || !layers || !layer_count || !rgb_planes || !out_planes)
+            return UNBLENDING_INVALID_ARGUMENT;
+        const size_t n = size_t(width) * size_t(height), L = layer_count;
+        // gpu_dispatch.cpp:136-158 planes -> packed, and back (:182-191).
+        std::vector<double> rgb(3 * n), first, out(4 * L * n);
+        for (size_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) rgb[3 * i + c] = rgb_planes[c * n + i];
+        if (first_planes) {
+            first.resize(4 * L * n);
+            for (size_t l = 0; l < L; ++l)
+                for (size_t i = 0; i < n; ++i) {
+                    double* px = first.data() + 4 * (l * n + i);
+                    for (int c = 0; c < 3; ++c) px[c] = first_planes[(L + 3 * l + c) * n + i];
+                    px[3] = first_planes[l * n + i];
+                }
+        }
+        unblending_rgb_view view{rgb.data(), rgb.size(), width, height, 0};
+        unblending_options opt = UNBLENDING_OPTIONS_INIT;
+        opt.backend = UNBLENDING_BACKEND_HIP;
+        opt.opaque_background = opaque ? 1 : 0;
+        opt.smooth_background = smooth ? 1 : 0;
+        const int32_t status = first_planes
+            ? unblending_refine(&view, layers, L, first.data(), first.size(), &opt, out.data(), out.size())
+            : unblending_unmix(&view, layers, L, &opt, out.data(), out.size());
+        if (status != UNBLENDING_OK) return status;
+        for (size_t l = 0; l < L; ++l)
+            for (size_t i = 0; i < n; ++i) {
+                const double* px = out.data() + 4 * (l * n + i);
+                for (int c = 0; c < 3; ++c) out_planes[(L + 3 * l + c) * n + i] = px[c];
+                out_planes[l * n + i] = px[3];
+            }
+        return UNBLENDING_OK;
+    } catch (const std::bad_alloc&) {
+        return UNBLENDING_OUT_OF_MEMORY;
+    }
+}
+
+UNBLENDING_API const char* unblending_hip_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// ub_host.hpp -- host-side pieces of the HIP backend shared by the C interface
+// (ub_capi.cpp) and the device pipeline (ub_pipeline.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "unblending.h"
+#include "unblending_hip.h"
+#include "ub_kernels.hpp"
+
+namespace ubhip {
+
+// A device failure; maps to UNBLENDING_DEVICE_ERROR as the reference's
+// gpu::ExecutionError maps to UNBLENDING_CUDA_ERROR (unblending_c.cpp:88).
+struct ExecutionError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+extern thread_local std::string g_last_error;
+
+// A validated layer set.
+struct LayerSet {
+    std::vector<unblending_layer> host;
+    bool all_plus = false, all_so = false;
+};
+
+LayerSet make_layer_set(const unblending_layer* layers, size_t count);
+size_t validate_pixels(unblending_rgb_view& image, size_t layer_count);
+
+// One device-resident pass (first pass when first == nullptr).
+struct DeviceJob {
+    int w = 0, h = 0;
+    const LayerSet* set = nullptr;
+    const double* rgb = nullptr;       // [n][3]
+    const double* first = nullptr;     // packed first pass (refinement)
+    const double* alpha_in = nullptr;  // optional alpha planes of the first pass
+    double* out = nullptr;             // packed output
+    double* alpha_out = nullptr;       // optional alpha planes of the output (first pass)
+    bool opaque = true, smooth = false;
+    unblending_hip_params params{};
+    void* workspace = nullptr;
+    size_t workspace_size = 0;
+    hipStream_t stream = nullptr;
+    unblending_hip_stats* stats = nullptr;
+};
+
+size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth);
+size_t filter_workspace_bytes(size_t n, int channels);
+void run_device(const DeviceJob& job);
+void run_filter(const double* rgb, int w, int h, const Channel* in, double* const* q, int channels, int radius,
+                double eps, double* ab, hipStream_t stream, unblending_hip_stats* stats, bool timing);
+void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const double* first, bool opaque,
+                bool smooth, const unblending_hip_params& params, double* output, unblending_hip_stats* stats);
+
+hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream);
+
+}  // namespace ubhip

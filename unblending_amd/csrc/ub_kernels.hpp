@@ -1,0 +1,69 @@
+// ub_kernels.hpp -- launch interface between the host pipeline and the gfx950
+// kernels (types shared by ub_solve.hip, ub_filter.hip and ub_pipeline.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "ub_device.hpp"
+
+namespace ubhip {
+
+constexpr int kMaxLayers = 8;      // layer counts with a compiled solve kernel
+constexpr int kLbfgsMemory = 4;    // (s, y) pairs of the inner minimiser
+
+// Solver constants, unblending/src/unblending.cpp:220-249, plus the inner
+// minimiser's own (line-search trials).
+struct SolverParams {
+    int max_evals;   // 1000
+    int max_count;   // 20: the outer loop stops when count > max_count
+    int max_ls;      // 20
+    int alpha_fixed; // refinement alphas fixed by bounds instead of constraints
+    double ftol, xtol, eps, gamma, beta, rho0;
+};
+
+// Per-pass work counters accumulated by the solve kernels.
+enum Counter : int { kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kNumCounters = 4 };
+
+struct PixelArgs {
+    DevLayer layers[kMaxLayers];  // passed by value; staged into LDS by the kernel
+    const double* rgb;     // [n][3] target colours
+    long long n;           // pixels
+    SolverParams sp;
+    int opaque, smooth, all_plus, all_so;
+    const double* first;   // refinement: first pass, packed [L][n][RGBA]
+    const double* filtered;// refinement: guided-filter output planes [L (+3)][n]
+    double* out;           // packed [L][n][RGBA]
+    double* alpha_planes;  // optional copy of the solved alphas [L][n]
+    unsigned long long* counters;
+};
+
+hipError_t launch_solve(int L, bool refine, const PixelArgs& args, hipStream_t stream);
+
+// One filtered channel: value(pixel) = base[pixel * stride].
+struct Channel {
+    const double* base;
+    int stride;
+};
+
+constexpr int kMaxGroup = 8;  // channels filtered by one launch
+
+struct FilterArgs {
+    const double* rgb;    // guidance [n][3]
+    int w, h, r;
+    double eps;
+    int groups;           // channels in this launch (<= kMaxGroup)
+    Channel in[kMaxGroup];// pass A inputs
+    double* ab;           // pass A output / pass B input: [4 * group][n] (a_r, a_g, a_b, b)
+    double* q[kMaxGroup]; // pass B outputs
+};
+
+// Pass A: box statistics and per-pixel coefficients; pass B: box of the
+// coefficients and the output q.  Returns hipErrorInvalidValue when the
+// radius is beyond the strip kernels' reach (r > kMaxFilterRadius).
+constexpr int kMaxFilterRadius = 480;
+hipError_t launch_filter_coefficients(const FilterArgs& args, hipStream_t stream);
+hipError_t launch_filter_output(const FilterArgs& args, hipStream_t stream);
+
+}  // namespace ubhip

@@ -1,0 +1,459 @@
+// ub_solve.hip -- per-pixel constrained solve on gfx950, one pixel per lane.
+//
+// Each lane runs, for its pixel, the reference's augmented-Lagrangian loop
+// (unblending/src/unblending.cpp:155-309: rho0 = 100, beta = 10, gamma = 0.25,
+// eps = 5e-3, at most 22 outer iterations, the last iterate returned) around
+// the project's projected L-BFGS inner minimiser (the stand-in for NLopt
+// LD_LBFGS at unblending.cpp:249: maxeval 1000, ftol_rel = xtol_rel = 5e-3):
+//   free set -> two-loop recursion -> projected Armijo backtracking with
+//   quadratic interpolation in [0.1 t, 0.5 t] -> (s, y) update with a
+//   curvature test -> NLopt's relative ftol / xtol tests.
+// The layer set is staged once per workgroup in LDS and read as a broadcast.
+// The L-BFGS history is kept newest-last and shifted on each update so every
+// register index is static.
+#include "ub_kernels.hpp"
+
+namespace ubhip {
+namespace {
+
+__device__ __forceinline__ double clampd(double v, double lo, double hi)
+{
+    v = (v < lo) ? lo : v;
+    return (v > hi) ? hi : v;
+}
+
+__device__ __forceinline__ bool relstop(double vold, double vnew, double reltol)
+{
+    return __builtin_fabs(vnew - vold) < reltol * (__builtin_fabs(vnew) + __builtin_fabs(vold)) * 0.5 ||
+           vnew == vold;
+}
+
+// Per-variable bounds (unblending.cpp:174-208): [0, 1]; alpha_0 = 1 when
+// opaque; alphas at their targets when fixed; background colour fixed when
+// smoothed.
+template <int L, bool REFINE>
+struct Bounds {
+    bool opaque, smooth, alpha_fixed;
+    const double* ta;
+    const double* bg;
+    __device__ __forceinline__ void get(int i, double& lo, double& hi) const
+    {
+        lo = 0.0;
+        hi = 1.0;
+        if (i < L) {
+            if (REFINE && alpha_fixed) { lo = ta[i]; hi = ta[i]; }
+            if (i == 0 && opaque) {
+                lo = 1.0;
+                if (REFINE && alpha_fixed) hi = 1.0;
+            }
+        } else if (REFINE && smooth && i < L + 3) {
+            lo = bg[i - L];
+            hi = bg[i - L];
+        }
+    }
+};
+
+template <int L, bool REFINE>
+struct Problem {
+    const DevLayer* lay;
+    double tgt[3];
+    double ta[L];
+    double lam[Shape<L, REFINE>::NC];
+    double rho;
+    __device__ __forceinline__ double eval(const double* x, double* g, double* C) const
+    {
+        return objective<L, REFINE>(x, lay, tgt, ta, lam, rho, g, C);
+    }
+};
+
+// The inner minimiser (the host oracle's orc_minimize, same arithmetic).
+template <int L, bool REFINE, int M>
+__device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L, REFINE>& pb,
+                         const SolverParams& sp, int& nev_out, int& iters_out)
+{
+    constexpr int N = 4 * L;
+    constexpr int NC = Shape<L, REFINE>::NC;
+    double g[N], d[N], q[N], xt[N], gt[N], C[NC];
+    double S[M][N], Yh[M][N], rh[M], a[M];
+    bool freev[N];
+    int count = 0;
+    double gamma = 1.0;
+
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double lo, hi;
+        bd.get(i, lo, hi);
+        x[i] = clampd(x[i], lo, hi);
+    }
+    double f = pb.eval(x, g, C);
+    int nev = 1, iters = 0;
+
+    for (;;) {
+        bool finite = __builtin_isfinite(f);
+#pragma unroll
+        for (int i = 0; i < N; ++i) finite = finite && __builtin_isfinite(g[i]);
+        if (!finite) break;
+
+        double pgmax = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double lo, hi;
+            bd.get(i, lo, hi);
+            freev[i] = lo < hi && !(x[i] <= lo && g[i] > 0.0) && !(x[i] >= hi && g[i] < 0.0);
+            q[i] = freev[i] ? g[i] : 0.0;
+            const double m = __builtin_fabs(q[i]);
+            pgmax = (m > pgmax) ? m : pgmax;
+        }
+        if (pgmax == 0.0) break;
+        ++iters;
+
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            if (M - 1 - k < count) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) s += S[k][i] * q[i];
+                a[k] = rh[k] * s;
+#pragma unroll
+                for (int i = 0; i < N; ++i) q[i] = q[i] - a[k] * Yh[k][i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = gamma * q[i];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            if (k >= M - count) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) s += Yh[k][i] * d[i];
+                const double b = rh[k] * s;
+#pragma unroll
+                for (int i = 0; i < N; ++i) d[i] = d[i] + (a[k] - b) * S[k][i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = freev[i] ? -d[i] : 0.0;
+        double gd = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) gd += g[i] * d[i];
+        if (!(gd < 0.0)) {
+            count = 0;
+            gamma = 1.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = freev[i] ? -g[i] : 0.0;
+        }
+
+        double t = 1.0;
+        if (count == 0) {
+            double dd = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) dd += d[i] * d[i];
+            const double inv = 1.0 / __builtin_sqrt(dd);
+            t = (inv < 1.0) ? inv : 1.0;
+        }
+
+        bool accepted = false;
+        double ft = 0.0;
+        for (int ls = 0;; ++ls) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double lo, hi;
+                bd.get(i, lo, hi);
+                xt[i] = clampd(x[i] + t * d[i], lo, hi);
+            }
+            ft = pb.eval(xt, gt, C);
+            ++nev;
+            double dphi = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) dphi += g[i] * (xt[i] - x[i]);
+            if (__builtin_isfinite(ft) && ft <= f + 1e-4 * dphi) {
+                accepted = true;
+                break;
+            }
+            if (nev >= sp.max_evals || ls + 1 >= sp.max_ls) break;
+            double tn;
+            if (dphi < 0.0) {
+                const double denom = 2.0 * (ft - f - dphi);
+                tn = (-dphi * t) / denom;
+            } else {
+                tn = 0.5 * t;
+            }
+            const double lo = 0.1 * t, hi = 0.5 * t;
+            tn = (tn >= lo) ? tn : lo;
+            tn = (tn > hi) ? hi : tn;
+            t = tn;
+        }
+        if (!accepted) break;
+
+        double sy = 0.0, yy = 0.0;
+        bool xstop = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double s = xt[i] - x[i];
+            const double y = gt[i] - g[i];
+            sy += s * y;
+            yy += y * y;
+            xstop = xstop && (__builtin_fabs(s) < sp.xtol * (__builtin_fabs(xt[i]) + __builtin_fabs(x[i])) * 0.5 ||
+                              xt[i] == x[i]);
+            x[i] = xt[i];
+            g[i] = gt[i];
+            xt[i] = s;  // keep s and y for the history update
+            gt[i] = y;
+        }
+        const double fold = f;
+        f = ft;
+        if (sy > 2.220446049250313e-16 * yy) {
+#pragma unroll
+            for (int k = 0; k + 1 < M; ++k) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    S[k][i] = S[k + 1][i];
+                    Yh[k][i] = Yh[k + 1][i];
+                }
+                rh[k] = rh[k + 1];
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                S[M - 1][i] = xt[i];
+                Yh[M - 1][i] = gt[i];
+            }
+            rh[M - 1] = 1.0 / sy;
+            gamma = sy / yy;
+            count = (count < M) ? count + 1 : M;
+        } else {
+            count = 0;
+            gamma = 1.0;
+        }
+        if (nev >= sp.max_evals) break;
+        if (relstop(fold, f, sp.ftol)) break;
+        if (xstop) break;
+    }
+    nev_out = nev;
+    iters_out = iters;
+}
+
+// Constraint vector alone (equations.cpp:113-155), for the outer loop.
+template <int L, bool REFINE>
+__device__ __forceinline__ void constraints(const double* x, const DevLayer* lay, const double tgt[3],
+                                            const double* ta, double* C)
+{
+    const double* alphas = x;
+    const double* colors = x + L;
+    double acc0 = colors[0], acc1 = colors[1], acc2 = colors[2], accA = alphas[0];
+#pragma unroll
+    for (int k = 1; k < L; ++k) {
+        const DevLayer& l = lay[k];
+        const int mode = uniform_mode(l.mode);
+        const double as = alphas[k], ad = accA;
+        const double a = l.X * as * ad + l.Y * as * (1.0 - ad) + l.Z * ad * (1.0 - as);
+        const double ky = l.Y * as * (1.0 - ad);
+        const double kz = l.Z * ad * (1.0 - as);
+        double cd[3] = {acc0, acc1, acc2};
+        double out[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double cs = colors[3 * k + c];
+            const double f = blend(cs, cd[c], mode);
+            const double p = f * as * ad + ky * cs + kz * cd[c];
+            out[c] = (a > 1e-12) ? p / a : p;
+        }
+        acc0 = out[0];
+        acc1 = out[1];
+        acc2 = out[2];
+        accA = a;
+    }
+    C[0] = acc0 - tgt[0];
+    C[1] = acc1 - tgt[1];
+    C[2] = acc2 - tgt[2];
+    if constexpr (REFINE) {
+#pragma unroll
+        for (int i = 0; i < L; ++i) C[3 + i] = alphas[i] - ta[i];
+    } else {
+        C[3] = accA - 1.0;
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ double norm(const double* v)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) s += v[j] * v[j];
+    return __builtin_sqrt(s);
+}
+
+template <int L, bool REFINE, int M>
+__global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
+{
+    constexpr int N = 4 * L;
+    constexpr int NC = Shape<L, REFINE>::NC;
+    __shared__ DevLayer lay[L];
+    if (threadIdx.x < L) lay[threadIdx.x] = args.layers[threadIdx.x];
+    __syncthreads();
+
+    const long long pix = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = pix < args.n;
+    const long long i = active ? pix : 0;
+
+    Problem<L, REFINE> pb;
+    pb.lay = lay;
+    for (int c = 0; c < 3; ++c) pb.tgt[c] = args.rgb[3 * i + c];
+    double bgc[3] = {0.0, 0.0, 0.0};
+    double x[N];
+    if constexpr (REFINE) {
+        // crop and normalise the filtered alphas (unblending.cpp:366-389, :311-333)
+        double sum = 0.0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            double v = args.filtered[(long long)l * args.n + i];
+            v = (v < 0.0) ? 0.0 : ((v > 1.0) ? 1.0 : v);
+            pb.ta[l] = v;
+        }
+        if (args.all_so && __builtin_fabs(pb.ta[0] - 1.0) < 1e-05) {
+        } else if (args.all_plus) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) sum += pb.ta[l];
+#pragma unroll
+            for (int l = 0; l < L; ++l) pb.ta[l] = pb.ta[l] / sum;
+        } else {
+#pragma unroll
+            for (int l = 0; l < L; ++l) pb.ta[l] = 0.0;
+        }
+        if (args.smooth) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double v = args.filtered[(long long)(L + c) * args.n + i];
+                bgc[c] = (v < 0.0) ? 0.0 : ((v > 1.0) ? 1.0 : v);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            x[l] = pb.ta[l];
+            const double* px = args.first + 4 * ((long long)l * args.n + i);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = px[c];
+        }
+    } else {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            pb.ta[l] = 0.0;
+            x[l] = 0.50;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = lay[l].mean[c];
+        }
+    }
+    if (args.opaque) x[0] = 1.0;
+    if (REFINE && args.smooth) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[L + c] = bgc[c];
+    }
+
+    Bounds<L, REFINE> bd;
+    bd.opaque = args.opaque != 0;
+    bd.smooth = args.smooth != 0;
+    bd.alpha_fixed = args.sp.alpha_fixed != 0;
+    bd.ta = pb.ta;
+    bd.bg = bgc;
+
+#pragma unroll
+    for (int j = 0; j < NC; ++j) pb.lam[j] = 0.0;
+    pb.rho = args.sp.rho0;
+
+    unsigned long long evals = 0, iters = 0, outer = 0;
+    if (active) {
+        int count = 0;
+        for (;;) {
+            double xn[N];
+#pragma unroll
+            for (int v = 0; v < N; ++v) xn[v] = x[v];
+            int ev = 0, it = 0;
+            minimize<L, REFINE, M>(xn, bd, pb, args.sp, ev, it);
+            evals += ev;
+            iters += it;
+            outer += 1;
+            double g[NC], gn[NC];
+            constraints<L, REFINE>(x, lay, pb.tgt, pb.ta, g);
+            constraints<L, REFINE>(xn, lay, pb.tgt, pb.ta, gn);
+#pragma unroll
+            for (int j = 0; j < NC; ++j) pb.lam[j] = pb.lam[j] - pb.rho * gn[j];
+            const double ngn = norm<NC>(gn);
+            if (ngn > args.sp.gamma * norm<NC>(g)) pb.rho = pb.rho * args.sp.beta;
+            double dx[N];
+#pragma unroll
+            for (int v = 0; v < N; ++v) dx[v] = xn[v] - x[v];
+            const bool unchanged = norm<N>(dx) < args.sp.eps;
+            const bool satisfied = ngn < args.sp.eps;
+#pragma unroll
+            for (int v = 0; v < N; ++v) x[v] = xn[v];
+            if ((unchanged && satisfied) || count > args.sp.max_count) break;
+            ++count;
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            double* px = args.out + 4 * ((long long)l * args.n + pix);
+            px[0] = x[L + 3 * l];
+            px[1] = x[L + 3 * l + 1];
+            px[2] = x[L + 3 * l + 2];
+            px[3] = x[l];
+            if (args.alpha_planes) args.alpha_planes[(long long)l * args.n + pix] = x[l];
+        }
+    }
+    if (args.counters) {
+        // wave-level reduction, one atomic per counter per wave
+        for (int off = 32; off > 0; off >>= 1) {
+            evals += __shfl_xor(evals, off);
+            iters += __shfl_xor(iters, off);
+            outer += __shfl_xor(outer, off);
+        }
+        const unsigned long long pixels = __popcll(__ballot(active));
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&args.counters[kEvals], evals);
+            atomicAdd(&args.counters[kIterations], iters);
+            atomicAdd(&args.counters[kOuter], outer);
+            atomicAdd(&args.counters[kPixels], pixels);
+        }
+    }
+}
+
+template <int L>
+hipError_t launch_L(bool refine, const PixelArgs& args, hipStream_t stream)
+{
+    const unsigned blocks = (unsigned)((args.n + 63) / 64);
+    if (refine)
+        hipLaunchKernelGGL((solve_simple_kernel<L, true, kLbfgsMemory>), dim3(blocks), dim3(64), 0, stream, args);
+    else
+        hipLaunchKernelGGL((solve_simple_kernel<L, false, kLbfgsMemory>), dim3(blocks), dim3(64), 0, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+__global__ void zero_kernel(unsigned long long* p, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0ull;
+}
+
+hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream)
+{
+    hipLaunchKernelGGL(zero_kernel, dim3((count + 63) / 64), dim3(64), 0, stream, p, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve(int L, bool refine, const PixelArgs& args, hipStream_t stream)
+{
+    if (args.n <= 0) return hipSuccess;
+    switch (L) {
+    case 1: return launch_L<1>(refine, args, stream);
+    case 2: return launch_L<2>(refine, args, stream);
+    case 3: return launch_L<3>(refine, args, stream);
+    case 4: return launch_L<4>(refine, args, stream);
+    case 5: return launch_L<5>(refine, args, stream);
+    case 6: return launch_L<6>(refine, args, stream);
+    case 7: return launch_L<7>(refine, args, stream);
+    case 8: return launch_L<8>(refine, args, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ubhip

@@ -1,0 +1,117 @@
+"""Device-resident calls over torch tensors on the current HIP device.
+
+Torch provides device memory and the stream; the work is done by
+libunblending.so's device entry points (include/unblending_hip.h), enqueued
+on torch's current stream so torch events time them.  Shapes:
+    rgb    (h, w, 3)     float64, contiguous
+    first  (L, h, w, 4)  float64 packed first pass
+    out    (L, h, w, 4)
+    alpha  (L, h, w)     optional alpha planes
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import (BACKEND_HIP, HipParams, HipStats, _lib, hip_params, layer_array, options, raise_for_status)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _check(t, shape, name):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("%s must be a contiguous float64 device tensor" % name)
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError("%s has shape %s, expected %s" % (name, tuple(t.shape), tuple(shape)))
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def workspace_size(w, h, L, refine, smooth=False) -> int:
+    return int(_lib.unblending_hip_workspace_size(int(w), int(h), int(L), int(bool(refine)), int(bool(smooth))))
+
+
+def workspace(w, h, L, refine, smooth=False, device=None):
+    torch = _torch()
+    n = max(1, (workspace_size(w, h, L, refine, smooth) + 7) // 8)
+    return torch.empty(n, dtype=torch.float64, device=device or "cuda")
+
+
+def unmix(rgb, layer_infos, opaque=True, out=None, alpha_planes=None, ws=None, params: HipParams | None = None,
+          stream=None, stats: HipStats | None = None):
+    torch = _torch()
+    h, w, _ = rgb.shape
+    L = len(layer_infos)
+    _check(rgb, (h, w, 3), "rgb")
+    if out is None:
+        out = torch.empty((L, h, w, 4), dtype=torch.float64, device=rgb.device)
+    _check(out, (L, h, w, 4), "out")
+    if alpha_planes is not None:
+        _check(alpha_planes, (L, h, w), "alpha_planes")
+    if ws is None:
+        ws = workspace(w, h, L, False, device=rgb.device)
+    opt = options(BACKEND_HIP, opaque, False)
+    st = _lib.unblending_hip_unmix_device(
+        ctypes.c_void_p(rgb.data_ptr()), w, h, layer_array(layer_infos), L, ctypes.byref(opt),
+        ctypes.byref(params or hip_params()), ctypes.c_void_p(out.data_ptr()),
+        ctypes.c_void_p(alpha_planes.data_ptr()) if alpha_planes is not None else None,
+        ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
+        ctypes.byref(stats) if stats is not None else None)
+    raise_for_status(st)
+    return out
+
+
+def refine(rgb, first, layer_infos, opaque=True, smooth=False, alpha_planes=None, out=None, ws=None,
+           params: HipParams | None = None, stream=None, stats: HipStats | None = None):
+    torch = _torch()
+    h, w, _ = rgb.shape
+    L = len(layer_infos)
+    _check(rgb, (h, w, 3), "rgb")
+    _check(first, (L, h, w, 4), "first")
+    if alpha_planes is not None:
+        _check(alpha_planes, (L, h, w), "alpha_planes")
+    if out is None:
+        out = torch.empty((L, h, w, 4), dtype=torch.float64, device=rgb.device)
+    _check(out, (L, h, w, 4), "out")
+    if ws is None:
+        ws = workspace(w, h, L, True, smooth, device=rgb.device)
+    opt = options(BACKEND_HIP, opaque, smooth)
+    st = _lib.unblending_hip_refine_device(
+        ctypes.c_void_p(rgb.data_ptr()), w, h, layer_array(layer_infos), L,
+        ctypes.c_void_p(first.data_ptr()),
+        ctypes.c_void_p(alpha_planes.data_ptr()) if alpha_planes is not None else None,
+        ctypes.byref(opt), ctypes.byref(params or hip_params()), ctypes.c_void_p(out.data_ptr()),
+        ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
+        ctypes.byref(stats) if stats is not None else None)
+    raise_for_status(st)
+    return out
+
+
+def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stats: HipStats | None = None):
+    """q[c] = guided filter of planes[c] (each (h, w)) with guidance rgb."""
+    torch = _torch()
+    h, w, _ = rgb.shape
+    _check(rgb, (h, w, 3), "rgb")
+    C = len(planes)
+    for i, p in enumerate(planes):
+        _check(p, (h, w), "planes[%d]" % i)
+    if out is None:
+        out = [torch.empty((h, w), dtype=torch.float64, device=rgb.device) for _ in range(C)]
+    if ws is None:
+        n = max(1, (int(_lib.unblending_hip_filter_workspace_size(w, h, C)) + 7) // 8)
+        ws = torch.empty(n, dtype=torch.float64, device=rgb.device)
+    pin = (ctypes.c_void_p * C)(*[p.data_ptr() for p in planes])
+    qout = (ctypes.c_void_p * C)(*[q.data_ptr() for q in out])
+    st = _lib.unblending_hip_guided_filter_device(
+        ctypes.c_void_p(rgb.data_ptr()), w, h, pin, qout, C, int(radius), float(eps),
+        ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
+        ctypes.byref(stats) if stats is not None else None)
+    raise_for_status(st)
+    return out
