@@ -38,7 +38,7 @@
 
 void orc_default_settings(orc_settings* s)
 {
-    s->memory = 4;
+    s->memory = 1;
     s->max_evals = 1000;   /* unblending.cpp:249 */
     s->max_count = 20;     /* unblending.cpp:246 */
     s->max_ls = 20;

@@ -29,12 +29,16 @@ def _workload(k, size):
     return synth.config(k, size)
 
 
+SOLVERS = [0, 1]  # persistent kernel, one-pixel-per-lane kernel
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("k,size", [(1, (64, 48)), (2, (40, 40)), (3, (32, 36))])
-def test_unmix_matches_oracle(oracle, k, size):
+def test_unmix_matches_oracle(oracle, k, size, solver):
     import unblending_amd as U
     wl = _workload(k, size)
-    print("case unmix c%d %s" % (k, size), flush=True)
-    gpu = U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True)
+    print("case unmix c%d %s solver %d" % (k, size, solver), flush=True)
+    gpu = U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True, params=U.hip_params(solver=solver))
     ref, _ = oracle.unmix(wl.rgb, oracle_layers(wl.layers), opaque=True)
     d = _report("unmix c%d" % k, gpu, ref)
     assert d.max() <= ALPHA_COLOR_TOL
@@ -43,14 +47,15 @@ def test_unmix_matches_oracle(oracle, k, size):
     assert np.sqrt(np.mean((rg - rr) ** 2)) <= RECOMPOSITE_RMS_TOL
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("k,size", [(1, (64, 48)), (2, (40, 40)), (3, (32, 36))])
-def test_refine_matches_oracle(oracle, k, size):
+def test_refine_matches_oracle(oracle, k, size, solver):
     import unblending_amd as U
     wl = _workload(k, size)
-    print("case refine c%d %s" % (k, size), flush=True)
+    print("case refine c%d %s solver %d" % (k, size, solver), flush=True)
     ol = oracle_layers(wl.layers)
     first, _ = oracle.unmix(wl.rgb, ol, opaque=True)
-    params = U.hip_params(radius=wl.radius, eps=wl.eps)
+    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=solver)
     gpu = U.perform_matte_refinement(wl.rgb, first, layer_infos(wl.layers), True, False, params=params)
     ref, _ = oracle.refine(wl.rgb, ol, first, opaque=True, radius=wl.radius, eps=wl.eps)
     d = _report("refine c%d" % k, gpu, ref)

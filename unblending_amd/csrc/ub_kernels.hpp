@@ -11,7 +11,7 @@
 namespace ubhip {
 
 constexpr int kMaxLayers = 8;      // layer counts with a compiled solve kernel
-constexpr int kLbfgsMemory = 4;    // (s, y) pairs of the inner minimiser
+constexpr int kLbfgsMemory = 1;    // (s, y) pairs of the inner minimiser
 
 // Solver constants, unblending/src/unblending.cpp:220-249, plus the inner
 // minimiser's own (line-search trials).
@@ -24,7 +24,8 @@ struct SolverParams {
 };
 
 // Per-pass work counters accumulated by the solve kernels.
-enum Counter : int { kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kNumCounters = 4 };
+// kQueue is the persistent solve's next-pixel counter.
+enum Counter : int { kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kNumCounters = 5 };
 
 struct PixelArgs {
     DevLayer layers[kMaxLayers];  // passed by value; staged into LDS by the kernel
@@ -32,6 +33,7 @@ struct PixelArgs {
     long long n;           // pixels
     SolverParams sp;
     int opaque, smooth, all_plus, all_so;
+    int solver;            // 0 = persistent kernel, 1 = one pixel per lane
     const double* first;   // refinement: first pass, packed [L][n][RGBA]
     const double* filtered;// refinement: guided-filter output planes [L (+3)][n]
     double* out;           // packed [L][n][RGBA]

@@ -193,6 +193,7 @@ void run_device(const DeviceJob& job)
     args.smooth = job.smooth ? 1 : 0;
     args.all_plus = job.set->all_plus ? 1 : 0;
     args.all_so = job.set->all_so ? 1 : 0;
+    args.solver = job.params.solver;
     args.out = job.out;
     args.alpha_planes = job.alpha_out;
     args.counters = counters;
@@ -222,7 +223,7 @@ void run_device(const DeviceJob& job)
     check(launch_solve(int(L), refine, args, job.stream), "solve");
     es.stop(job.stream);
     if (job.stats) {
-        unsigned long long host_counters[kNumCounters] = {0, 0, 0, 0};
+        unsigned long long host_counters[kNumCounters] = {0, 0, 0, 0, 0};
         check(hipMemcpyAsync(host_counters, counters, sizeof host_counters, hipMemcpyDeviceToHost, job.stream),
               "counters");
         check(hipStreamSynchronize(job.stream), "hipStreamSynchronize");

@@ -282,53 +282,44 @@ __device__ __forceinline__ double norm(const double* v)
     return __builtin_sqrt(s);
 }
 
-template <int L, bool REFINE, int M>
-__global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
+// Per-pixel inputs and the initial solution (unblending.cpp:185-208, 401-428):
+// target colour; refinement targets = crop + normalisation of the filtered
+// alphas (unblending.cpp:366-389, 311-333) and the cropped smoothed
+// background; x = default or refinement initialisation.
+template <int L, bool REFINE>
+__device__ __forceinline__ void load_pixel(const PixelArgs& args, const DevLayer* lay, long long i, double tgt[3],
+                                           double ta[L], double bg[3], double* x)
 {
-    constexpr int N = 4 * L;
-    constexpr int NC = Shape<L, REFINE>::NC;
-    __shared__ DevLayer lay[L];
-    if (threadIdx.x < L) lay[threadIdx.x] = args.layers[threadIdx.x];
-    __syncthreads();
-
-    const long long pix = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = pix < args.n;
-    const long long i = active ? pix : 0;
-
-    Problem<L, REFINE> pb;
-    pb.lay = lay;
-    for (int c = 0; c < 3; ++c) pb.tgt[c] = args.rgb[3 * i + c];
-    double bgc[3] = {0.0, 0.0, 0.0};
-    double x[N];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tgt[c] = args.rgb[3 * i + c];
+    bg[0] = bg[1] = bg[2] = 0.0;
     if constexpr (REFINE) {
-        // crop and normalise the filtered alphas (unblending.cpp:366-389, :311-333)
-        double sum = 0.0;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             double v = args.filtered[(long long)l * args.n + i];
-            v = (v < 0.0) ? 0.0 : ((v > 1.0) ? 1.0 : v);
-            pb.ta[l] = v;
+            ta[l] = (v < 0.0) ? 0.0 : ((v > 1.0) ? 1.0 : v);
         }
-        if (args.all_so && __builtin_fabs(pb.ta[0] - 1.0) < 1e-05) {
+        if (args.all_so && __builtin_fabs(ta[0] - 1.0) < 1e-05) {
         } else if (args.all_plus) {
+            double sum = 0.0;
 #pragma unroll
-            for (int l = 0; l < L; ++l) sum += pb.ta[l];
+            for (int l = 0; l < L; ++l) sum += ta[l];
 #pragma unroll
-            for (int l = 0; l < L; ++l) pb.ta[l] = pb.ta[l] / sum;
+            for (int l = 0; l < L; ++l) ta[l] = ta[l] / sum;
         } else {
 #pragma unroll
-            for (int l = 0; l < L; ++l) pb.ta[l] = 0.0;
+            for (int l = 0; l < L; ++l) ta[l] = 0.0;
         }
         if (args.smooth) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                double v = args.filtered[(long long)(L + c) * args.n + i];
-                bgc[c] = (v < 0.0) ? 0.0 : ((v > 1.0) ? 1.0 : v);
+                const double v = args.filtered[(long long)(L + c) * args.n + i];
+                bg[c] = (v < 0.0) ? 0.0 : ((v > 1.0) ? 1.0 : v);
             }
         }
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            x[l] = pb.ta[l];
+            x[l] = ta[l];
             const double* px = args.first + 4 * ((long long)l * args.n + i);
 #pragma unroll
             for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = px[c];
@@ -336,7 +327,7 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
     } else {
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            pb.ta[l] = 0.0;
+            ta[l] = 0.0;
             x[l] = 0.50;
 #pragma unroll
             for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = lay[l].mean[c];
@@ -345,8 +336,26 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
     if (args.opaque) x[0] = 1.0;
     if (REFINE && args.smooth) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x[L + c] = bgc[c];
+        for (int c = 0; c < 3; ++c) x[L + c] = bg[c];
     }
+}
+
+template <int L, bool REFINE, int M>
+__global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
+{
+    constexpr int N = 4 * L;
+    constexpr int NC = Shape<L, REFINE>::NC;
+    const DevLayer* lay = args.layers;  // uniform: scalar loads from the argument segment
+
+    const long long pix = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = pix < args.n;
+    const long long i = active ? pix : 0;
+
+    Problem<L, REFINE> pb;
+    pb.lay = lay;
+    double bgc[3];
+    double x[N];
+    load_pixel<L, REFINE>(args, lay, i, pb.tgt, pb.ta, bgc, x);
 
     Bounds<L, REFINE> bd;
     bd.opaque = args.opaque != 0;
@@ -415,9 +424,371 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent, evaluation-synchronous solve.
+//
+// Every trip around the kernel's loop evaluates the objective once for every
+// lane; between evaluations each lane advances its own pixel's state machine
+// (start of an inner solve, line-search trial, search direction, outer
+// update, next pixel).  A lane whose pixel is finished takes the next pixel
+// index from a device counter, so lanes of one wave stay busy however uneven
+// the per-pixel iteration counts are.  The arithmetic is the same sequence,
+// per pixel, as solve_simple_kernel and the host oracle.
+//
+// Registers hold the current point, gradient and the L-BFGS pairs; LDS holds,
+// per lane, the outer iterate, the search direction and the multipliers.
+// History slot 0 doubles as the trial point / trial gradient: it is free or
+// holds the oldest pair, which the direction no longer needs once computed.
+// ---------------------------------------------------------------------------
+enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kIdle = 3 };
+
+template <int L, bool REFINE, int M>
+__global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
+{
+    constexpr int N = 4 * L;
+    constexpr int NC = Shape<L, REFINE>::NC;
+    const DevLayer* lay = args.layers;  // uniform: scalar loads from the argument segment
+    __shared__ double s_xo[N][64];
+    __shared__ double s_d[N][64];
+    __shared__ double s_lam[NC][64];
+    __shared__ double s_c[NC][64];
+    const int lane = threadIdx.x;
+
+    const SolverParams sp = args.sp;
+    const bool opaque = args.opaque != 0;
+    const bool smooth = REFINE && args.smooth != 0;
+    const bool afixed = REFINE && sp.alpha_fixed != 0;
+
+    double xi[N], g[N], S[M][N], Y[M][N], rh[M];
+    double tgt[3], ta[L], bg[3];
+    double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
+    int phase = kNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
+    long long pix = 0;
+    unsigned long long c_evals = 0, c_iters = 0, c_outer = 0, c_pixels = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        xi[i] = 0.0;
+        g[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            S[k][i] = 0.0;
+            Y[k][i] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) rh[k] = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) ta[l] = 0.0;
+    tgt[0] = tgt[1] = tgt[2] = 0.0;
+    bg[0] = bg[1] = bg[2] = 0.0;
+
+    auto bounds = [&](int i, double& lo, double& hi) {
+        lo = 0.0;
+        hi = 1.0;
+        if (i < L) {
+            if (afixed) { lo = ta[i]; hi = ta[i]; }
+            if (i == 0 && opaque) {
+                lo = 1.0;
+                if (afixed) hi = 1.0;
+            }
+        } else if (smooth && i < L + 3) {
+            lo = bg[i - L];
+            hi = bg[i - L];
+        }
+    };
+
+    for (;;) {
+        // ---- next pixels for the lanes that finished theirs ----
+        const unsigned long long needy = __ballot(phase == kNeedPixel);
+        if (needy) {
+            const int leader = __ffsll((long long)needy) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&args.counters[kQueue], (unsigned long long)__popcll(needy));
+            base = __shfl(base, leader);
+            if (phase == kNeedPixel) {
+                pix = (long long)(base + (unsigned long long)__popcll(needy & ((1ull << lane) - 1ull)));
+                if (pix >= args.n) {
+                    phase = kIdle;
+                } else {
+                    double x0[N], C0[NC];
+                    load_pixel<L, REFINE>(args, lay, pix, tgt, ta, bg, x0);
+#pragma unroll
+                    for (int j = 0; j < NC; ++j) s_lam[j][lane] = 0.0;
+                    rho = sp.rho0;
+                    ocount = 0;
+                    constraints<L, REFINE>(x0, lay, tgt, ta, C0);
+                    cn_outer = norm<NC>(C0);
+                    count = 0;
+                    gamma = 1.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        s_xo[i][lane] = x0[i];
+                        double lo, hi;
+                        bounds(i, lo, hi);
+                        S[0][i] = clampd(x0[i], lo, hi);
+                    }
+                    phase = kStart;
+                }
+            }
+        }
+        if (__ballot(phase != kIdle) == 0) break;
+
+        // ---- one evaluation per lane at history slot 0 ----
+        double lam[NC], Ct[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) lam[j] = s_lam[j][lane];
+        const double ft = objective<L, REFINE>(S[0], lay, tgt, ta, lam, rho, Y[0], Ct);
+
+        bool direction = false, inner_end = false;
+        if (phase == kStart) {
+            nev = 1;
+            f = ft;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                xi[i] = S[0][i];
+                g[i] = Y[0][i];
+            }
+#pragma unroll
+            for (int j = 0; j < NC; ++j) s_c[j][lane] = Ct[j];
+            direction = true;
+        } else if (phase == kTrial) {
+            ++nev;
+            double dphi = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) dphi += g[i] * (S[0][i] - xi[i]);
+            if (__builtin_isfinite(ft) && ft <= f + 1e-4 * dphi) {
+                double sy = 0.0, yy = 0.0;
+                bool xstop = true;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const double xt = S[0][i], gt = Y[0][i];
+                    const double sv = xt - xi[i];
+                    const double yv = gt - g[i];
+                    sy += sv * yv;
+                    yy += yv * yv;
+                    xstop = xstop && (__builtin_fabs(sv) < sp.xtol * (__builtin_fabs(xt) + __builtin_fabs(xi[i])) * 0.5 ||
+                                      xt == xi[i]);
+                    xi[i] = xt;
+                    g[i] = gt;
+                    S[0][i] = sv;
+                    Y[0][i] = yv;
+                }
+                const double fold = f;
+                f = ft;
+#pragma unroll
+                for (int j = 0; j < NC; ++j) s_c[j][lane] = Ct[j];
+                if (sy > 2.220446049250313e-16 * yy) {
+                    rh[0] = 1.0 / sy;
+                    gamma = sy / yy;
+                    // slot 0 (newest) -> slot M-1, the rest move one down
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        const double s0 = S[0][i], y0 = Y[0][i];
+#pragma unroll
+                        for (int k = 0; k + 1 < M; ++k) {
+                            S[k][i] = S[k + 1][i];
+                            Y[k][i] = Y[k + 1][i];
+                        }
+                        S[M - 1][i] = s0;
+                        Y[M - 1][i] = y0;
+                    }
+                    const double r0 = rh[0];
+#pragma unroll
+                    for (int k = 0; k + 1 < M; ++k) rh[k] = rh[k + 1];
+                    rh[M - 1] = r0;
+                    count = (count < M) ? count + 1 : M;
+                } else {
+                    count = 0;
+                    gamma = 1.0;
+                }
+                if (nev >= sp.max_evals || relstop(fold, f, sp.ftol) || xstop) inner_end = true;
+                else direction = true;
+            } else if (nev >= sp.max_evals || ls + 1 >= sp.max_ls) {
+                inner_end = true;
+            } else {
+                double tn;
+                if (dphi < 0.0) {
+                    const double denom = 2.0 * (ft - f - dphi);
+                    tn = (-dphi * t) / denom;
+                } else {
+                    tn = 0.5 * t;
+                }
+                const double lo = 0.1 * t, hi = 0.5 * t;
+                tn = (tn >= lo) ? tn : lo;
+                tn = (tn > hi) ? hi : tn;
+                t = tn;
+                ++ls;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    double blo, bhi;
+                    bounds(i, blo, bhi);
+                    S[0][i] = clampd(xi[i] + t * s_d[i][lane], blo, bhi);
+                }
+            }
+        }
+
+        if (direction) {
+            bool finite = __builtin_isfinite(f);
+#pragma unroll
+            for (int i = 0; i < N; ++i) finite = finite && __builtin_isfinite(g[i]);
+            unsigned int freev = 0;
+            double q[N];
+            double pgmax = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double lo, hi;
+                bounds(i, lo, hi);
+                const bool fr = lo < hi && !(xi[i] <= lo && g[i] > 0.0) && !(xi[i] >= hi && g[i] < 0.0);
+                freev |= fr ? (1u << i) : 0u;
+                q[i] = fr ? g[i] : 0.0;
+                const double m = __builtin_fabs(q[i]);
+                pgmax = (m > pgmax) ? m : pgmax;
+            }
+            if (!finite || pgmax == 0.0) {
+                inner_end = true;
+            } else {
+                ++c_iters;
+                double a[M];
+#pragma unroll
+                for (int k = M - 1; k >= 0; --k) {
+                    a[k] = 0.0;
+                    if (M - 1 - k < count) {
+                        double sd = 0.0;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) sd += S[k][i] * q[i];
+                        a[k] = rh[k] * sd;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) q[i] = q[i] - a[k] * Y[k][i];
+                    }
+                }
+                double dv[N];
+#pragma unroll
+                for (int i = 0; i < N; ++i) dv[i] = gamma * q[i];
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    if (k >= M - count) {
+                        double sd = 0.0;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) sd += Y[k][i] * dv[i];
+                        const double b = rh[k] * sd;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) dv[i] = dv[i] + (a[k] - b) * S[k][i];
+                    }
+                }
+                double gd = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    dv[i] = ((freev >> i) & 1u) ? -dv[i] : 0.0;
+                    gd += g[i] * dv[i];
+                }
+                if (!(gd < 0.0)) {
+                    count = 0;
+                    gamma = 1.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) dv[i] = ((freev >> i) & 1u) ? -g[i] : 0.0;
+                }
+                t = 1.0;
+                if (count == 0) {
+                    double dd = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) dd += dv[i] * dv[i];
+                    const double inv = 1.0 / __builtin_sqrt(dd);
+                    t = (inv < 1.0) ? inv : 1.0;
+                }
+                ls = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    s_d[i][lane] = dv[i];
+                    double lo, hi;
+                    bounds(i, lo, hi);
+                    S[0][i] = clampd(xi[i] + t * dv[i], lo, hi);
+                }
+                phase = kTrial;
+            }
+        }
+
+        if (inner_end) {
+            c_evals += (unsigned long long)nev;
+            c_outer += 1;
+            double Ccur[NC];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                Ccur[j] = s_c[j][lane];
+                s_lam[j][lane] = s_lam[j][lane] - rho * Ccur[j];
+            }
+            const double ngn = norm<NC>(Ccur);
+            if (ngn > sp.gamma * cn_outer) rho = rho * sp.beta;
+            double dx2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const double dxv = xi[i] - s_xo[i][lane];
+                dx2 += dxv * dxv;
+                s_xo[i][lane] = xi[i];
+            }
+            const bool unchanged = __builtin_sqrt(dx2) < sp.eps;
+            const bool satisfied = ngn < sp.eps;
+            cn_outer = ngn;
+            if ((unchanged && satisfied) || ocount > sp.max_count) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    double* px = args.out + 4 * ((long long)l * args.n + pix);
+                    px[0] = xi[L + 3 * l];
+                    px[1] = xi[L + 3 * l + 1];
+                    px[2] = xi[L + 3 * l + 2];
+                    px[3] = xi[l];
+                    if (args.alpha_planes) args.alpha_planes[(long long)l * args.n + pix] = xi[l];
+                }
+                ++c_pixels;
+                phase = kNeedPixel;
+            } else {
+                ++ocount;
+                count = 0;
+                gamma = 1.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    double lo, hi;
+                    bounds(i, lo, hi);
+                    S[0][i] = clampd(xi[i], lo, hi);
+                }
+                phase = kStart;
+            }
+        }
+    }
+
+    for (int off = 32; off > 0; off >>= 1) {
+        c_evals += __shfl_xor(c_evals, off);
+        c_iters += __shfl_xor(c_iters, off);
+        c_outer += __shfl_xor(c_outer, off);
+        c_pixels += __shfl_xor(c_pixels, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&args.counters[kEvals], c_evals);
+        atomicAdd(&args.counters[kIterations], c_iters);
+        atomicAdd(&args.counters[kOuter], c_outer);
+        atomicAdd(&args.counters[kPixels], c_pixels);
+    }
+}
+
+template <int L, bool REFINE>
+hipError_t launch_persistent(const PixelArgs& args, hipStream_t stream)
+{
+    auto kernel = solve_persistent_kernel<L, REFINE, kLbfgsMemory>;
+    int device = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0);
+    if (e != hipSuccess) return e;
+    long long blocks = (long long)cus * (per_cu > 0 ? per_cu : 1);
+    const long long needed = (args.n + 63) / 64;
+    if (blocks > needed) blocks = needed;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(64), 0, stream, args);
+    return hipGetLastError();
+}
+
 template <int L>
 hipError_t launch_L(bool refine, const PixelArgs& args, hipStream_t stream)
 {
+    if (args.solver == 0) return refine ? launch_persistent<L, true>(args, stream) : launch_persistent<L, false>(args, stream);
     const unsigned blocks = (unsigned)((args.n + 63) / 64);
     if (refine)
         hipLaunchKernelGGL((solve_simple_kernel<L, true, kLbfgsMemory>), dim3(blocks), dim3(64), 0, stream, args);
