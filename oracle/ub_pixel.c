@@ -338,8 +338,10 @@ int orc_constraint_jacobian(const double* x, int L, const orc_layer* layers, int
 }
 
 /* unblending.cpp:67-114.  f = E + (-lambda.C) + 0.5 rho |C|^2 and
- * grad = dE + J (rho C - lambda), where J (rho C - lambda) is accumulated by a
- * reverse sweep through the same 4x4 blocks as orc_constraint_jacobian. */
+ * grad = dE + J (rho C - lambda).  J (rho C - lambda) is accumulated by a
+ * reverse sweep through the same 4x4 blocks as orc_constraint_jacobian, with
+ * the block entries' common divisor A applied as one reciprocal per layer.
+ * The energy terms are computed during the forward sweep. */
 double orc_objective(const double* x, int L, const orc_layer* layers, const double target[3],
                      const double* target_alphas, const double* lambda, double rho,
                      double* grad, double* C)
@@ -348,14 +350,28 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
     const double* alphas = x;
     const double* colors = x + L;
 
-    /* forward composite, keeping every prefix (equations.cpp:56-71) */
+    /* forward composite keeping every prefix (equations.cpp:56-71), with the
+     * energy (equations.cpp:88-111) and its colour gradient (:168-175) */
     double pre[ORC_MAX_LAYERS][4];
-    pre[0][0] = colors[0];
-    pre[0][1] = colors[1];
-    pre[0][2] = colors[2];
-    pre[0][3] = alphas[0];
-    for (int k = 1; k < L; ++k)
-        orc_composite_two(colors + 3 * k, pre[k - 1], alphas[k], pre[k - 1][3], &layers[k], pre[k]);
+    double dist[ORC_MAX_LAYERS];
+    double energy = 0.0;
+    for (int k = 0; k < L; ++k) {
+        const double* ck = colors + 3 * k;
+        if (k == 0) {
+            pre[0][0] = ck[0];
+            pre[0][1] = ck[1];
+            pre[0][2] = ck[2];
+            pre[0][3] = alphas[0];
+        } else {
+            orc_composite_two(ck, pre[k - 1], alphas[k], pre[k - 1][3], &layers[k], pre[k]);
+        }
+        dist[k] = orc_distance(ck, &layers[k]);
+        energy += alphas[k] * dist[k];
+        double g[3];
+        distance_gradient(ck, &layers[k], g);
+        if (grad)
+            for (int j = 0; j < 3; ++j) grad[L + 3 * k + j] = alphas[k] * g[j];
+    }
 
     /* constraints (equations.cpp:113-155) */
     int nc;
@@ -366,14 +382,6 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
     } else {
         C[3] = pre[L - 1][3] - 1.0;
         nc = 4;
-    }
-
-    /* energy (equations.cpp:88-111) and its gradient (:157-196) */
-    double energy = 0.0;
-    double dist[ORC_MAX_LAYERS];
-    for (int i = 0; i < L; ++i) {
-        dist[i] = orc_distance(colors + 3 * i, &layers[i]);
-        energy += alphas[i] * dist[i];
     }
     if (!refine) {
         double sum = 0.0;
@@ -390,7 +398,9 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
     double w[ORC_MAX_CONSTRAINTS];
     for (int j = 0; j < nc; ++j) w[j] = rho * C[j] - lambda[j];
 
-    /* reverse sweep: adjoint of the composite (B, A) */
+    /* reverse sweep: adjoint of the composite (B, A).  The alpha block of dE
+     * is D(c_i) in refinement and the reference's constant 0.01 in the first
+     * pass (equations.cpp:189-193). */
     double aB[3] = {w[0], w[1], w[2]};
     double aA = refine ? 0.0 : w[3];
     for (int k = L - 1; k >= 1; --k) {
@@ -398,7 +408,7 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
         const double* cs = colors + 3 * k;
         const double* cd = pre[k - 1];
         const double as = alphas[k], ad = pre[k - 1][3];
-        const double A = pre[k][3];
+        const double invA = 1.0 / pre[k][3];
         const double Y = (double)l->comp_y, Z = (double)l->comp_z;
         const double pA_s = dA_das(ad, l);
         const double pA_d = dA_dad(as, l);
@@ -409,35 +419,26 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
             const double gs = orc_blend_grad_s(cs[c], cd[c], l->blend_mode);
             const double gd = orc_blend_grad_d(cs[c], cd[c], l->blend_mode);
             const double B = pre[k][c];
-            const double dB_dcs = (as * ad * gs + Y * (1.0 - ad) * as) / A;
-            const double dB_das = ((D * ad + Y * (1.0 - ad) * cs[c] - Z * ad * cd[c]) - B * pA_s) / A;
-            const double dB_dcd = (as * ad * gd + Z * (1.0 - as) * ad) / A;
-            const double dB_dad = ((D * as - Y * as * cs[c] + Z * (1.0 - as) * cd[c]) - B * pA_d) / A;
-            grad[L + 3 * k + c] = dB_dcs * aB[c];
+            const double dB_dcs = (as * ad * gs + Y * (1.0 - ad) * as) * invA;
+            const double dB_das = ((D * ad + Y * (1.0 - ad) * cs[c] - Z * ad * cd[c]) - B * pA_s) * invA;
+            const double dB_dcd = (as * ad * gd + Z * (1.0 - as) * ad) * invA;
+            const double dB_dad = ((D * as - Y * as * cs[c] + Z * (1.0 - as) * cd[c]) - B * pA_d) * invA;
+            grad[L + 3 * k + c] = grad[L + 3 * k + c] + dB_dcs * aB[c];
             adj_as += dB_das * aB[c];
             adj_ad += dB_dad * aB[c];
             nB[c] = dB_dcd * aB[c];
         }
-        grad[k] = adj_as + pA_s * aA;
+        double ga = adj_as + pA_s * aA;
+        if (refine) ga = ga + w[3 + k];
+        grad[k] = (refine ? dist[k] : 0.01) + ga;
         aA = adj_ad + pA_d * aA;
         aB[0] = nB[0];
         aB[1] = nB[1];
         aB[2] = nB[2];
     }
-    grad[0] = aA;
-    grad[L + 0] = aB[0];
-    grad[L + 1] = aB[1];
-    grad[L + 2] = aB[2];
-    if (refine)
-        for (int i = 0; i < L; ++i) grad[i] += w[3 + i];
-
-    /* + dE (the reference adds dE and J w elementwise, unblending.cpp:100) */
-    for (int i = 0; i < L; ++i) {
-        const double ge = refine ? dist[i] : 0.01;
-        grad[i] = ge + grad[i];
-        double g[3];
-        distance_gradient(colors + 3 * i, &layers[i], g);
-        for (int j = 0; j < 3; ++j) grad[L + 3 * i + j] = alphas[i] * g[j] + grad[L + 3 * i + j];
-    }
+    double ga0 = aA;
+    if (refine) ga0 = ga0 + w[3];
+    grad[0] = (refine ? dist[0] : 0.01) + ga0;
+    for (int c = 0; c < 3; ++c) grad[L + c] = grad[L + c] + aB[c];
     return f;
 }

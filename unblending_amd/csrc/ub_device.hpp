@@ -69,75 +69,118 @@ __device__ __forceinline__ double blend(double s, double d, int mode)
     }
 }
 
-// Blend value and both partial derivatives in one pass (the three switch
-// statements of blend_mode.hpp:90-196 share their branch conditions).
-__device__ __forceinline__ void blend_all(double s, double d, int mode, double& D, double& gs, double& gd)
+// Blend value and partial derivatives of one channel, without per-lane
+// branches: both sides of every condition of blend_mode.hpp:90-196 are
+// evaluated and the result is selected, which yields the same values as the
+// branching form (an unselected side may divide by zero; it is discarded).
+template <int MODE>
+__device__ __forceinline__ void blend_sel(double s, double d, double& D, double& gs, double& gd)
 {
-    switch (mode) {
-    case kNormal: D = s; gs = 1.0; gd = 0.0; return;
-    case kMultiply: D = s * d; gs = d; gd = s; return;
-    case kScreen: D = 1.0 - (1.0 - s) * (1.0 - d); gs = 1.0 - d; gd = 1.0 - s; return;
-    case kOverlay:
-        if (d <= 0.5) { D = 2.0 * s * d; gs = 2.0 * d; gd = 2.0 * s; }
-        else { D = 1.0 - 2.0 * (1.0 - s) * (1.0 - d); gs = 2.0 * (1.0 - d); gd = 2.0 * (1.0 - s); }
-        return;
-    case kDarken:
-        if (s < d) { D = s; gs = 1.0; gd = 0.0; } else { D = d; gs = 0.0; gd = 1.0; }
-        return;
-    case kLighten:
-        if (s < d) { D = d; gs = 0.0; gd = 1.0; } else { D = s; gs = 1.0; gd = 0.0; }
-        return;
-    case kColorDodge:
-        if (d < kBlendEps) { D = 0.0; gs = 0.0; gd = 0.0; return; }
-        if (1.0 - s < kBlendEps) { D = 1.0; gs = 0.0; gd = 0.0; return; }
-        {
-            const double q = d / (1.0 - s);
-            D = (q < 1.0) ? q : 1.0;
-            if (1.0 < q) { gs = 0.0; gd = 0.0; }
-            else { gs = d / ((1.0 - s) * (1.0 - s)); gd = 1.0 / (1.0 - s); }
-        }
-        return;
-    case kColorBurn:
-        if (1.0 - d < kBlendEps) { D = 1.0; gs = 0.0; gd = 0.0; return; }
-        if (s < kBlendEps) { D = 0.0; gs = 0.0; gd = 0.0; return; }
-        {
-            const double q = (1.0 - d) / s;
-            D = 1.0 - ((q < 1.0) ? q : 1.0);
-            if (1.0 < q) { gs = -0.0; gd = -0.0; }
-            else { gs = -(-(1.0 - d) / (s * s)); gd = -(-1.0 / s); }
-        }
-        return;
-    case kHardLight:
-        if (s <= 0.5) { D = 2.0 * s * d; gs = 2.0 * d; gd = 2.0 * s; }
-        else { D = 1.0 - 2.0 * (1.0 - s) * (1.0 - d); gs = 2.0 * (1.0 - d); gd = 2.0 * (1.0 - s); }
-        return;
-    case kSoftLight:
-        if (s <= 0.5) {
-            D = d - (1.0 - 2.0 * s) * d * (1.0 - d);
-            gs = 2.0 * d * (1.0 - d);
-            gd = 2.0 * s + 2.0 * d - 4.0 * s * d;
+    if constexpr (MODE == kNormal) {
+        D = s; gs = 1.0; gd = 0.0;
+    } else if constexpr (MODE == kMultiply) {
+        D = s * d; gs = d; gd = s;
+    } else if constexpr (MODE == kScreen) {
+        D = 1.0 - (1.0 - s) * (1.0 - d); gs = 1.0 - d; gd = 1.0 - s;
+    } else if constexpr (MODE == kOverlay || MODE == kHardLight) {
+        const bool lo = (MODE == kOverlay) ? (d <= 0.5) : (s <= 0.5);
+        const double D1 = 2.0 * s * d, D2 = 1.0 - 2.0 * (1.0 - s) * (1.0 - d);
+        D = lo ? D1 : D2;
+        if constexpr (MODE == kOverlay) {
+            gs = lo ? 2.0 * d : 2.0 * (1.0 - d);
+            gd = lo ? 2.0 * s : 2.0 * (1.0 - s);
         } else {
-            double g, h;
-            if (d <= 0.25) {
-                g = ((16.0 * d - 12.0) * d + 4.0) * d;
-                h = 48.0 * d * d - 24.0 * d + 4.0;
-            } else {
-                const double r = __builtin_sqrt(d);
-                g = r;
-                h = 1.0 / (2.0 * r);
-            }
-            D = d + (2.0 * s - 1.0) * (g - d);
-            gs = 2.0 * (g - d);
-            gd = 1.0 + (2.0 * s - 1.0) * h - (2.0 * s - 1.0);
+            gs = lo ? 2.0 * d : 2.0 * (1.0 - d);
+            gd = lo ? 2.0 * s : 2.0 * (1.0 - s);
         }
-        return;
-    case kDifference:
-        if (s < d) { D = d - s; gs = -1.0; gd = 1.0; } else { D = s - d; gs = 1.0; gd = -1.0; }
-        return;
-    case kExclusion: D = s + d - 2.0 * s * d; gs = 1.0 - 2.0 * d; gd = 1.0 - 2.0 * s; return;
-    case kLinearDodge: D = s + d; gs = 1.0; gd = 1.0; return;
-    default: D = 0.0; gs = 0.0; gd = 0.0; return;
+    } else if constexpr (MODE == kDarken) {
+        const bool lt = s < d;
+        D = lt ? s : d; gs = lt ? 1.0 : 0.0; gd = lt ? 0.0 : 1.0;
+    } else if constexpr (MODE == kLighten) {
+        const bool lt = s < d;
+        D = lt ? d : s; gs = lt ? 0.0 : 1.0; gd = lt ? 1.0 : 0.0;
+    } else if constexpr (MODE == kColorDodge) {
+        const bool c1 = d < kBlendEps, c2 = 1.0 - s < kBlendEps;
+        const double q = d / (1.0 - s);
+        const double mid = (q < 1.0) ? q : 1.0;
+        const bool flat = 1.0 < q;
+        const double gsm = flat ? 0.0 : d / ((1.0 - s) * (1.0 - s));
+        const double gdm = flat ? 0.0 : 1.0 / (1.0 - s);
+        D = c1 ? 0.0 : (c2 ? 1.0 : mid);
+        gs = (c1 || c2) ? 0.0 : gsm;
+        gd = (c1 || c2) ? 0.0 : gdm;
+    } else if constexpr (MODE == kColorBurn) {
+        const bool c1 = 1.0 - d < kBlendEps, c2 = s < kBlendEps;
+        const double q = (1.0 - d) / s;
+        const double mid = 1.0 - ((q < 1.0) ? q : 1.0);
+        const bool flat = 1.0 < q;
+        const double gsm = flat ? -0.0 : -(-(1.0 - d) / (s * s));
+        const double gdm = flat ? -0.0 : -(-1.0 / s);
+        D = c1 ? 1.0 : (c2 ? 0.0 : mid);
+        gs = (c1 || c2) ? 0.0 : gsm;
+        gd = (c1 || c2) ? 0.0 : gdm;
+    } else if constexpr (MODE == kSoftLight) {
+        const bool lo = s <= 0.5, small = d <= 0.25;
+        const double r = __builtin_sqrt(d);
+        const double g = small ? ((16.0 * d - 12.0) * d + 4.0) * d : r;
+        const double h = small ? (48.0 * d * d - 24.0 * d + 4.0) : 1.0 / (2.0 * r);
+        D = lo ? d - (1.0 - 2.0 * s) * d * (1.0 - d) : d + (2.0 * s - 1.0) * (g - d);
+        gs = lo ? 2.0 * d * (1.0 - d) : 2.0 * (g - d);
+        gd = lo ? 2.0 * s + 2.0 * d - 4.0 * s * d : 1.0 + (2.0 * s - 1.0) * h - (2.0 * s - 1.0);
+    } else if constexpr (MODE == kDifference) {
+        const bool lt = s < d;
+        D = lt ? d - s : s - d; gs = lt ? -1.0 : 1.0; gd = lt ? 1.0 : -1.0;
+    } else if constexpr (MODE == kExclusion) {
+        D = s + d - 2.0 * s * d; gs = 1.0 - 2.0 * d; gd = 1.0 - 2.0 * s;
+    } else if constexpr (MODE == kLinearDodge) {
+        D = s + d; gs = 1.0; gd = 1.0;
+    } else {
+        D = 0.0; gs = 0.0; gd = 0.0;
     }
+}
+
+// The three channels of one layer under one (uniform) mode: a single scalar
+// branch per layer, the channels as independent work inside it.  With
+// WANT_GRAD false only the blend value is produced.
+template <bool WANT_GRAD>
+__device__ __forceinline__ void blend3_impl(int mode, const double* s, const double* d, double* D, double* gs,
+                                            double* gd)
+{
+    double tgs[3], tgd[3];
+    double* ogs = WANT_GRAD ? gs : tgs;
+    double* ogd = WANT_GRAD ? gd : tgd;
+#define UB_CASE(M) \
+    case M: _Pragma("unroll") for (int c = 0; c < 3; ++c) blend_sel<M>(s[c], d[c], D[c], ogs[c], ogd[c]); return;
+    switch (mode) {
+        UB_CASE(kNormal)
+        UB_CASE(kMultiply)
+        UB_CASE(kScreen)
+        UB_CASE(kOverlay)
+        UB_CASE(kDarken)
+        UB_CASE(kLighten)
+        UB_CASE(kColorDodge)
+        UB_CASE(kColorBurn)
+        UB_CASE(kHardLight)
+        UB_CASE(kSoftLight)
+        UB_CASE(kDifference)
+        UB_CASE(kExclusion)
+        UB_CASE(kLinearDodge)
+    default:
+        _Pragma("unroll") for (int c = 0; c < 3; ++c) { D[c] = 0.0; ogs[c] = 0.0; ogd[c] = 0.0; }
+        return;
+    }
+#undef UB_CASE
+}
+
+__device__ __forceinline__ void blend3(int mode, const double* s, const double* d, double* D)
+{
+    blend3_impl<false>(mode, s, d, D, nullptr, nullptr);
+}
+
+__device__ __forceinline__ void blend3_all(int mode, const double* s, const double* d, double* D, double* gs,
+                                           double* gd)
+{
+    blend3_impl<true>(mode, s, d, D, gs, gd);
 }
 
 // (c - mu)^T P (c - mu), row vector times matrix first (color_model.hpp:29-31).
@@ -166,6 +209,16 @@ __device__ __forceinline__ int uniform_mode(int mode)
     return __builtin_amdgcn_readfirstlane(mode);
 }
 
+// The layer table is uniform and read with scalar loads.  Indexing it through
+// an opaque zero pins each layer's loads next to their use: otherwise the
+// scheduler hoists all layers' parameters together and spills the SGPRs.
+__device__ __forceinline__ const DevLayer& layer_at(const DevLayer* lay, int k)
+{
+    int zero;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+    return lay[k + zero];
+}
+
 // Number of constraints: 4 in the first pass, 3 + L in refinement.
 template <int L, bool REFINE>
 struct Shape {
@@ -174,7 +227,10 @@ struct Shape {
 };
 
 // Augmented-Lagrangian objective, its gradient and the constraint vector at x.
-// x = [alpha_0..alpha_{L-1}, c_0 rgb .. c_{L-1} rgb].
+// x = [alpha_0..alpha_{L-1}, c_0 rgb .. c_{L-1} rgb].  The forward sweep
+// composites bottom to top and accumulates the energy and its colour
+// gradient; the reverse sweep applies the transposed 4x4 blocks of
+// equations.cpp:222-282 (one reciprocal of A per layer) to rho C - lambda.
 template <int L, bool REFINE>
 __device__ __forceinline__ double objective(const double* __restrict__ x, const DevLayer* lay,
                                             const double tgt[3], const double* ta,
@@ -186,26 +242,38 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     const double* colors = x + L;
 
     double pre[L][4];
-    pre[0][0] = colors[0];
-    pre[0][1] = colors[1];
-    pre[0][2] = colors[2];
-    pre[0][3] = alphas[0];
+    double dist[L];
+    double energy = 0.0;
 #pragma unroll
-    for (int k = 1; k < L; ++k) {
-        const DevLayer& l = lay[k];
-        const int mode = uniform_mode(l.mode);
-        const double as = alphas[k], ad = pre[k - 1][3];
-        const double a = l.X * as * ad + l.Y * as * (1.0 - ad) + l.Z * ad * (1.0 - as);
-        const double ky = l.Y * as * (1.0 - ad);
-        const double kz = l.Z * ad * (1.0 - as);
+    for (int k = 0; k < L; ++k) {
+        const DevLayer& l = layer_at(lay, k);
+        const double* ck = colors + 3 * k;
+        if (k == 0) {
+            pre[0][0] = ck[0];
+            pre[0][1] = ck[1];
+            pre[0][2] = ck[2];
+            pre[0][3] = alphas[0];
+        } else {
+            const int mode = uniform_mode(l.mode);
+            const double as = alphas[k], ad = pre[k - 1][3];
+            const double a = l.X * as * ad + l.Y * as * (1.0 - ad) + l.Z * ad * (1.0 - as);
+            const double ky = l.Y * as * (1.0 - ad);
+            const double kz = l.Z * ad * (1.0 - as);
+            double f[3];
+            blend3(mode, ck, pre[k - 1], f);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double cs = colors[3 * k + c], cd = pre[k - 1][c];
-            const double f = blend(cs, cd, mode);
-            const double p = f * as * ad + ky * cs + kz * cd;
-            pre[k][c] = (a > 1e-12) ? p / a : p;
+            for (int c = 0; c < 3; ++c) {
+                const double p = f[c] * as * ad + ky * ck[c] + kz * pre[k - 1][c];
+                pre[k][c] = (a > 1e-12) ? p / a : p;
+            }
+            pre[k][3] = a;
         }
-        pre[k][3] = a;
+        dist[k] = distance(ck[0], ck[1], ck[2], l);
+        energy += alphas[k] * dist[k];
+        double g[3];
+        distance_gradient(ck[0], ck[1], ck[2], l, g);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) grad[L + 3 * k + j] = alphas[k] * g[j];
     }
 
 #pragma unroll
@@ -215,16 +283,6 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         for (int i = 0; i < L; ++i) C[3 + i] = alphas[i] - ta[i];
     } else {
         C[3] = pre[L - 1][3] - 1.0;
-    }
-
-    double energy = 0.0;
-    double dist[L];
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-        dist[i] = distance(colors[3 * i], colors[3 * i + 1], colors[3 * i + 2], lay[i]);
-        energy += alphas[i] * dist[i];
-    }
-    if constexpr (!REFINE) {
         double sum = 0.0;
 #pragma unroll
         for (int i = 0; i < L; ++i) sum += alphas[i];
@@ -241,57 +299,47 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
 #pragma unroll
     for (int j = 0; j < NC; ++j) w[j] = rho * C[j] - lam[j];
 
-    double aB0 = w[0], aB1 = w[1], aB2 = w[2];
+    double aB[3] = {w[0], w[1], w[2]};
     double aA = REFINE ? 0.0 : w[3 < NC ? 3 : 0];
 #pragma unroll
     for (int k = L - 1; k >= 1; --k) {
-        const DevLayer& l = lay[k];
+        const DevLayer& l = layer_at(lay, k);
         const int mode = uniform_mode(l.mode);
+        const double* cs = colors + 3 * k;
+        const double* cd = pre[k - 1];
         const double as = alphas[k], ad = pre[k - 1][3];
-        const double A = pre[k][3];
+        const double invA = 1.0 / pre[k][3];
         const double pA_s = l.X * ad + l.Y * (1.0 - ad) - l.Z * ad;
         const double pA_d = l.X * as - l.Y * as + l.Z * (1.0 - as);
+        double D[3], gs[3], gd[3];
+        blend3_all(mode, cs, cd, D, gs, gd);
         double adj_as = 0.0, adj_ad = 0.0;
-        double aB[3] = {aB0, aB1, aB2};
         double nB[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double cs = colors[3 * k + c], cd = pre[k - 1][c];
-            double D, gs, gd;
-            blend_all(cs, cd, mode, D, gs, gd);
             const double B = pre[k][c];
-            const double dB_dcs = (as * ad * gs + l.Y * (1.0 - ad) * as) / A;
-            const double dB_das = ((D * ad + l.Y * (1.0 - ad) * cs - l.Z * ad * cd) - B * pA_s) / A;
-            const double dB_dcd = (as * ad * gd + l.Z * (1.0 - as) * ad) / A;
-            const double dB_dad = ((D * as - l.Y * as * cs + l.Z * (1.0 - as) * cd) - B * pA_d) / A;
-            grad[L + 3 * k + c] = dB_dcs * aB[c];
+            const double dB_dcs = (as * ad * gs[c] + l.Y * (1.0 - ad) * as) * invA;
+            const double dB_das = ((D[c] * ad + l.Y * (1.0 - ad) * cs[c] - l.Z * ad * cd[c]) - B * pA_s) * invA;
+            const double dB_dcd = (as * ad * gd[c] + l.Z * (1.0 - as) * ad) * invA;
+            const double dB_dad = ((D[c] * as - l.Y * as * cs[c] + l.Z * (1.0 - as) * cd[c]) - B * pA_d) * invA;
+            grad[L + 3 * k + c] = grad[L + 3 * k + c] + dB_dcs * aB[c];
             adj_as += dB_das * aB[c];
             adj_ad += dB_dad * aB[c];
             nB[c] = dB_dcd * aB[c];
         }
-        grad[k] = adj_as + pA_s * aA;
+        double ga = adj_as + pA_s * aA;
+        if constexpr (REFINE) ga = ga + w[3 + k];
+        grad[k] = (REFINE ? dist[k] : 0.01) + ga;
         aA = adj_ad + pA_d * aA;
-        aB0 = nB[0];
-        aB1 = nB[1];
-        aB2 = nB[2];
+        aB[0] = nB[0];
+        aB[1] = nB[1];
+        aB[2] = nB[2];
     }
-    grad[0] = aA;
-    grad[L + 0] = aB0;
-    grad[L + 1] = aB1;
-    grad[L + 2] = aB2;
-    if constexpr (REFINE) {
+    double ga0 = aA;
+    if constexpr (REFINE) ga0 = ga0 + w[3];
+    grad[0] = (REFINE ? dist[0] : 0.01) + ga0;
 #pragma unroll
-        for (int i = 0; i < L; ++i) grad[i] += w[3 + i];
-    }
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-        const double ge = REFINE ? dist[i] : 0.01;
-        grad[i] = ge + grad[i];
-        double g[3];
-        distance_gradient(colors[3 * i], colors[3 * i + 1], colors[3 * i + 2], lay[i], g);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) grad[L + 3 * i + j] = alphas[i] * g[j] + grad[L + 3 * i + j];
-    }
+    for (int c = 0; c < 3; ++c) grad[L + c] = grad[L + c] + aB[c];
     return f;
 }
 

@@ -12,6 +12,8 @@ namespace ubhip {
 
 constexpr int kMaxLayers = 8;      // layer counts with a compiled solve kernel
 constexpr int kLbfgsMemory = 1;    // (s, y) pairs of the inner minimiser
+constexpr int kMaxPersistentBlocks = 2048;  // 64-lane blocks of the persistent solve
+constexpr long long kLaneStateDoubles = (long long)kMaxPersistentBlocks * 64 * 4 * kMaxLayers;
 
 // Solver constants, unblending/src/unblending.cpp:220-249, plus the inner
 // minimiser's own (line-search trials).
@@ -38,6 +40,7 @@ struct PixelArgs {
     const double* filtered;// refinement: guided-filter output planes [L (+3)][n]
     double* out;           // packed [L][n][RGBA]
     double* alpha_planes;  // optional copy of the solved alphas [L][n]
+    double* lane_state;    // persistent solve: kLaneStateDoubles of per-lane scratch
     unsigned long long* counters;
 };
 

@@ -121,7 +121,7 @@ size_t filter_workspace_bytes(size_t n, int channels)
 
 size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth)
 {
-    size_t bytes = kHeader;
+    size_t bytes = kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
     if (refine) {
         const int C = filter_channels(layers, smooth);
         bytes += filter_workspace_bytes(n, C);
@@ -197,6 +197,7 @@ void run_device(const DeviceJob& job)
     args.out = job.out;
     args.alpha_planes = job.alpha_out;
     args.counters = counters;
+    args.lane_state = reinterpret_cast<double*>(ws + kHeader);
     check(launch_zero(counters, kNumCounters, job.stream), "zero counters");
 
     if (job.stats) *job.stats = unblending_hip_stats{};
@@ -205,8 +206,9 @@ void run_device(const DeviceJob& job)
         if (radius < 0) radius = 60 * std::min(job.w, job.h) / 1000;  // unblending.cpp:351
         const double eps = job.params.filter_epsilon > 0.0 ? job.params.filter_epsilon : 1e-4;  // :353
         const int C = filter_channels(L, job.smooth);
-        double* ab = reinterpret_cast<double*>(ws + kHeader);
-        double* qbase = reinterpret_cast<double*>(ws + kHeader + filter_workspace_bytes(n, C));
+        const size_t base = kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
+        double* ab = reinterpret_cast<double*>(ws + base);
+        double* qbase = reinterpret_cast<double*>(ws + base + filter_workspace_bytes(n, C));
         std::vector<Channel> in(C);
         std::vector<double*> q(C);
         for (size_t l = 0; l < L; ++l)

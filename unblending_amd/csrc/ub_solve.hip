@@ -435,31 +435,41 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // the per-pixel iteration counts are.  The arithmetic is the same sequence,
 // per pixel, as solve_simple_kernel and the host oracle.
 //
-// Registers hold the current point, gradient and the L-BFGS pairs; LDS holds,
-// per lane, the outer iterate, the search direction and the multipliers.
-// History slot 0 doubles as the trial point / trial gradient: it is free or
-// holds the oldest pair, which the direction no longer needs once computed.
+// Registers hold the current point and the (s, y) pair; the pair's registers
+// hold the trial point and trial gradient between a search direction and its
+// acceptance (the pair is no longer needed once the direction is computed).
+// LDS holds, per lane, the gradient, the search direction, the multipliers
+// and the constraint vector.  Only the outer iterate, read and written once
+// per outer iteration, has a per-lane global slot; state touched every
+// evaluation stays on chip, where it costs no memory traffic.
 // ---------------------------------------------------------------------------
 enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kIdle = 3 };
 
 template <int L, bool REFINE, int M>
 __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
 {
+    static_assert(M == 1, "the persistent kernel keeps one (s, y) pair");
     constexpr int N = 4 * L;
     constexpr int NC = Shape<L, REFINE>::NC;
-    const DevLayer* lay = args.layers;  // uniform: scalar loads from the argument segment
-    __shared__ double s_xo[N][64];
-    __shared__ double s_d[N][64];
-    __shared__ double s_lam[NC][64];
-    __shared__ double s_c[NC][64];
+    __shared__ double s_g[N][64];    // gradient at the current point
+    __shared__ double s_d[N][64];    // search direction
+    __shared__ double s_lam[NC][64]; // multipliers
+    __shared__ double s_c[NC][64];   // constraint vector at the current point
     const int lane = threadIdx.x;
+    // the outer iterate, touched once per outer iteration: a per-lane global
+    // slot, [variable][lane]
+    const long long stride = (long long)gridDim.x * 64;
+    double* xo = args.lane_state + (long long)blockIdx.x * 64 + lane;
 
     const SolverParams sp = args.sp;
     const bool opaque = args.opaque != 0;
     const bool smooth = REFINE && args.smooth != 0;
     const bool afixed = REFINE && sp.alpha_fixed != 0;
 
-    double xi[N], g[N], S[M][N], Y[M][N], rh[M];
+    // S, Y: the (s, y) pair, and between direction and acceptance the trial
+    // point and trial gradient.
+    double xc[N], S[N], Y[N];
+    double rh = 0.0;
     double tgt[3], ta[L], bg[3];
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
     int phase = kNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
@@ -467,16 +477,11 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
     unsigned long long c_evals = 0, c_iters = 0, c_outer = 0, c_pixels = 0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        xi[i] = 0.0;
-        g[i] = 0.0;
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-            S[k][i] = 0.0;
-            Y[k][i] = 0.0;
-        }
+        xc[i] = 0.0;
+        s_g[i][lane] = 0.0;
+        S[i] = 0.0;
+        Y[i] = 0.0;
     }
-#pragma unroll
-    for (int k = 0; k < M; ++k) rh[k] = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) ta[l] = 0.0;
     tgt[0] = tgt[1] = tgt[2] = 0.0;
@@ -498,6 +503,14 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
     };
 
     for (;;) {
+        // The layer table is uniform: it is read with scalar loads from the
+        // argument segment.  An opaque zero offset keeps those loads inside
+        // the loop (served by the scalar cache) instead of letting them be
+        // hoisted into ~200 loop-long SGPRs that would spill.
+        int zero;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+        const DevLayer* lay = args.layers + zero;
+
         // ---- next pixels for the lanes that finished theirs ----
         const unsigned long long needy = __ballot(phase == kNeedPixel);
         if (needy) {
@@ -522,10 +535,10 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                     gamma = 1.0;
 #pragma unroll
                     for (int i = 0; i < N; ++i) {
-                        s_xo[i][lane] = x0[i];
+                        xo[i * stride] = x0[i];
                         double lo, hi;
                         bounds(i, lo, hi);
-                        S[0][i] = clampd(x0[i], lo, hi);
+                        S[i] = clampd(x0[i], lo, hi);
                     }
                     phase = kStart;
                 }
@@ -533,11 +546,11 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         }
         if (__ballot(phase != kIdle) == 0) break;
 
-        // ---- one evaluation per lane at history slot 0 ----
+        // ---- one evaluation per lane at the trial point S ----
         double lam[NC], Ct[NC];
 #pragma unroll
         for (int j = 0; j < NC; ++j) lam[j] = s_lam[j][lane];
-        const double ft = objective<L, REFINE>(S[0], lay, tgt, ta, lam, rho, Y[0], Ct);
+        const double ft = objective<L, REFINE>(S, lay, tgt, ta, lam, rho, Y, Ct);
 
         bool direction = false, inner_end = false;
         if (phase == kStart) {
@@ -545,8 +558,8 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             f = ft;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                xi[i] = S[0][i];
-                g[i] = Y[0][i];
+                xc[i] = S[i];
+                s_g[i][lane] = Y[i];
             }
 #pragma unroll
             for (int j = 0; j < NC; ++j) s_c[j][lane] = Ct[j];
@@ -555,48 +568,32 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             ++nev;
             double dphi = 0.0;
 #pragma unroll
-            for (int i = 0; i < N; ++i) dphi += g[i] * (S[0][i] - xi[i]);
+            for (int i = 0; i < N; ++i) dphi += s_g[i][lane] * (S[i] - xc[i]);
             if (__builtin_isfinite(ft) && ft <= f + 1e-4 * dphi) {
                 double sy = 0.0, yy = 0.0;
                 bool xstop = true;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    const double xt = S[0][i], gt = Y[0][i];
-                    const double sv = xt - xi[i];
-                    const double yv = gt - g[i];
+                    const double xt = S[i], gt = Y[i];
+                    const double sv = xt - xc[i];
+                    const double yv = gt - s_g[i][lane];
                     sy += sv * yv;
                     yy += yv * yv;
-                    xstop = xstop && (__builtin_fabs(sv) < sp.xtol * (__builtin_fabs(xt) + __builtin_fabs(xi[i])) * 0.5 ||
-                                      xt == xi[i]);
-                    xi[i] = xt;
-                    g[i] = gt;
-                    S[0][i] = sv;
-                    Y[0][i] = yv;
+                    xstop = xstop && (__builtin_fabs(sv) < sp.xtol * (__builtin_fabs(xt) + __builtin_fabs(xc[i])) * 0.5 ||
+                                      xt == xc[i]);
+                    xc[i] = xt;
+                    s_g[i][lane] = gt;
+                    S[i] = sv;
+                    Y[i] = yv;
                 }
                 const double fold = f;
                 f = ft;
 #pragma unroll
                 for (int j = 0; j < NC; ++j) s_c[j][lane] = Ct[j];
                 if (sy > 2.220446049250313e-16 * yy) {
-                    rh[0] = 1.0 / sy;
+                    rh = 1.0 / sy;
                     gamma = sy / yy;
-                    // slot 0 (newest) -> slot M-1, the rest move one down
-#pragma unroll
-                    for (int i = 0; i < N; ++i) {
-                        const double s0 = S[0][i], y0 = Y[0][i];
-#pragma unroll
-                        for (int k = 0; k + 1 < M; ++k) {
-                            S[k][i] = S[k + 1][i];
-                            Y[k][i] = Y[k + 1][i];
-                        }
-                        S[M - 1][i] = s0;
-                        Y[M - 1][i] = y0;
-                    }
-                    const double r0 = rh[0];
-#pragma unroll
-                    for (int k = 0; k + 1 < M; ++k) rh[k] = rh[k + 1];
-                    rh[M - 1] = r0;
-                    count = (count < M) ? count + 1 : M;
+                    count = 1;
                 } else {
                     count = 0;
                     gamma = 1.0;
@@ -622,25 +619,26 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 for (int i = 0; i < N; ++i) {
                     double blo, bhi;
                     bounds(i, blo, bhi);
-                    S[0][i] = clampd(xi[i] + t * s_d[i][lane], blo, bhi);
+                    S[i] = clampd(xc[i] + t * s_d[i][lane], blo, bhi);
                 }
             }
         }
 
         if (direction) {
+            // free set and projected gradient; q is then turned into d in place
             bool finite = __builtin_isfinite(f);
-#pragma unroll
-            for (int i = 0; i < N; ++i) finite = finite && __builtin_isfinite(g[i]);
             unsigned int freev = 0;
             double q[N];
             double pgmax = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
+                const double gi = s_g[i][lane];
+                finite = finite && __builtin_isfinite(gi);
                 double lo, hi;
                 bounds(i, lo, hi);
-                const bool fr = lo < hi && !(xi[i] <= lo && g[i] > 0.0) && !(xi[i] >= hi && g[i] < 0.0);
+                const bool fr = lo < hi && !(xc[i] <= lo && gi > 0.0) && !(xc[i] >= hi && gi < 0.0);
                 freev |= fr ? (1u << i) : 0u;
-                q[i] = fr ? g[i] : 0.0;
+                q[i] = fr ? gi : 0.0;
                 const double m = __builtin_fabs(q[i]);
                 pgmax = (m > pgmax) ? m : pgmax;
             }
@@ -648,60 +646,52 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 inner_end = true;
             } else {
                 ++c_iters;
-                double a[M];
+                double a = 0.0;
+                if (count > 0) {
+                    double sd = 0.0;
 #pragma unroll
-                for (int k = M - 1; k >= 0; --k) {
-                    a[k] = 0.0;
-                    if (M - 1 - k < count) {
-                        double sd = 0.0;
+                    for (int i = 0; i < N; ++i) sd += S[i] * q[i];
+                    a = rh * sd;
 #pragma unroll
-                        for (int i = 0; i < N; ++i) sd += S[k][i] * q[i];
-                        a[k] = rh[k] * sd;
-#pragma unroll
-                        for (int i = 0; i < N; ++i) q[i] = q[i] - a[k] * Y[k][i];
-                    }
+                    for (int i = 0; i < N; ++i) q[i] = q[i] - a * Y[i];
                 }
-                double dv[N];
 #pragma unroll
-                for (int i = 0; i < N; ++i) dv[i] = gamma * q[i];
+                for (int i = 0; i < N; ++i) q[i] = gamma * q[i];
+                if (count > 0) {
+                    double sd = 0.0;
 #pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    if (k >= M - count) {
-                        double sd = 0.0;
+                    for (int i = 0; i < N; ++i) sd += Y[i] * q[i];
+                    const double b = rh * sd;
 #pragma unroll
-                        for (int i = 0; i < N; ++i) sd += Y[k][i] * dv[i];
-                        const double b = rh[k] * sd;
-#pragma unroll
-                        for (int i = 0; i < N; ++i) dv[i] = dv[i] + (a[k] - b) * S[k][i];
-                    }
+                    for (int i = 0; i < N; ++i) q[i] = q[i] + (a - b) * S[i];
                 }
                 double gd = 0.0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    dv[i] = ((freev >> i) & 1u) ? -dv[i] : 0.0;
-                    gd += g[i] * dv[i];
+                    q[i] = ((freev >> i) & 1u) ? -q[i] : 0.0;
+                    gd += s_g[i][lane] * q[i];
                 }
                 if (!(gd < 0.0)) {
                     count = 0;
                     gamma = 1.0;
 #pragma unroll
-                    for (int i = 0; i < N; ++i) dv[i] = ((freev >> i) & 1u) ? -g[i] : 0.0;
+                    for (int i = 0; i < N; ++i) q[i] = ((freev >> i) & 1u) ? -s_g[i][lane] : 0.0;
                 }
                 t = 1.0;
                 if (count == 0) {
                     double dd = 0.0;
 #pragma unroll
-                    for (int i = 0; i < N; ++i) dd += dv[i] * dv[i];
+                    for (int i = 0; i < N; ++i) dd += q[i] * q[i];
                     const double inv = 1.0 / __builtin_sqrt(dd);
                     t = (inv < 1.0) ? inv : 1.0;
                 }
                 ls = 0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    s_d[i][lane] = dv[i];
+                    s_d[i][lane] = q[i];
                     double lo, hi;
                     bounds(i, lo, hi);
-                    S[0][i] = clampd(xi[i] + t * dv[i], lo, hi);
+                    S[i] = clampd(xc[i] + t * q[i], lo, hi);
                 }
                 phase = kTrial;
             }
@@ -721,9 +711,9 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             double dx2 = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const double dxv = xi[i] - s_xo[i][lane];
+                const double dxv = xc[i] - xo[i * stride];
                 dx2 += dxv * dxv;
-                s_xo[i][lane] = xi[i];
+                xo[i * stride] = xc[i];
             }
             const bool unchanged = __builtin_sqrt(dx2) < sp.eps;
             const bool satisfied = ngn < sp.eps;
@@ -732,11 +722,11 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
                     double* px = args.out + 4 * ((long long)l * args.n + pix);
-                    px[0] = xi[L + 3 * l];
-                    px[1] = xi[L + 3 * l + 1];
-                    px[2] = xi[L + 3 * l + 2];
-                    px[3] = xi[l];
-                    if (args.alpha_planes) args.alpha_planes[(long long)l * args.n + pix] = xi[l];
+                    px[0] = xc[L + 3 * l];
+                    px[1] = xc[L + 3 * l + 1];
+                    px[2] = xc[L + 3 * l + 2];
+                    px[3] = xc[l];
+                    if (args.alpha_planes) args.alpha_planes[(long long)l * args.n + pix] = xc[l];
                 }
                 ++c_pixels;
                 phase = kNeedPixel;
@@ -748,7 +738,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 for (int i = 0; i < N; ++i) {
                     double lo, hi;
                     bounds(i, lo, hi);
-                    S[0][i] = clampd(xi[i], lo, hi);
+                    S[i] = clampd(xc[i], lo, hi);
                 }
                 phase = kStart;
             }
@@ -781,6 +771,7 @@ hipError_t launch_persistent(const PixelArgs& args, hipStream_t stream)
     long long blocks = (long long)cus * (per_cu > 0 ? per_cu : 1);
     const long long needed = (args.n + 63) / 64;
     if (blocks > needed) blocks = needed;
+    if (blocks > kMaxPersistentBlocks) blocks = kMaxPersistentBlocks;
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(64), 0, stream, args);
     return hipGetLastError();
 }
