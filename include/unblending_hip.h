@@ -41,11 +41,13 @@ typedef struct unblending_hip_params {
     int32_t alpha_fixed;      /* refinement: 1 = alphas fixed at the filtered targets by bounds */
     int32_t solver;           /* 0 = default kernel, 1 = simple one-pixel-per-lane kernel */
     int32_t timing;           /* 1 = record per-kernel device time in the stats (synchronises) */
-    int32_t reserved;
+    int32_t warm_start;       /* first pass: 1 = start from the model's prediction (mlp), re-solve
+                                 from the default initialisation where that misses the tolerances */
+    const void* mlp;          /* device buffer of a trained model (unblending_hip_mlp_*) */
 } unblending_hip_params;
 
 #define UNBLENDING_HIP_PARAMS_INIT \
-    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0 }
+    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL }
 
 typedef struct unblending_hip_stats {
     uint64_t evals;        /* objective+gradient evaluations, summed over pixels */
@@ -60,6 +62,8 @@ typedef struct unblending_hip_stats {
     int32_t filter_channels;
     int32_t filter_launches;
     int32_t reserved;
+    uint64_t fallback_pixels; /* warm start: pixels re-solved from the default initialisation */
+    double predict_ms;        /* warm start: device time of the model's prediction (timing = 1) */
 } unblending_hip_stats;
 
 UNBLENDING_API int32_t unblending_hip_unmix_ex(
@@ -73,10 +77,11 @@ UNBLENDING_API int32_t unblending_hip_refine_ex(
     const unblending_hip_params* params, double* output, size_t output_capacity,
     unblending_hip_stats* stats);
 
-/* Bytes of device workspace the device-resident calls need. */
+/* Bytes of device workspace the device-resident calls need (warm_start: a
+ * first pass started from the model's prediction). */
 UNBLENDING_API size_t unblending_hip_workspace_size(int32_t width, int32_t height,
                                                    size_t layer_count, int32_t refine,
-                                                   int32_t smooth_background);
+                                                   int32_t smooth_background, int32_t warm_start);
 
 UNBLENDING_API int32_t unblending_hip_unmix_device(
     const double* rgb, int32_t width, int32_t height, const unblending_layer* layers,
@@ -100,6 +105,36 @@ UNBLENDING_API int32_t unblending_hip_guided_filter_device(
 
 UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_t height,
                                                           int32_t channels);
+
+/* ---- warm-start model (no counterpart in the reference) ----
+ * A 27 -> 128 -> 128 -> 128 -> outputs network (GELU, sigmoid outputs) whose
+ * input is a pixel's 3x3 RGB neighbourhood and whose outputs are the packed
+ * first-pass values (alpha, R, G, B per layer; outputs = 4 * layers <= 32).
+ * BF16 matrix-core products with FP32 accumulation; FP32 master weights and
+ * Adam moments, all in one device buffer of unblending_hip_mlp_model_size()
+ * bytes.  Parameter vector (FP32, padded): W1 [32][128], b1, W2 [128][128],
+ * b2, W3 [128][128], b3, W4 [128][32], b4; weights [in][out]. */
+UNBLENDING_API size_t unblending_hip_mlp_model_size(void);
+UNBLENDING_API int32_t unblending_hip_mlp_param_count(void);  /* padded FP32 parameters */
+/* Uploads FP32 parameters, zeroes the Adam moments, makes the BF16 copies. */
+UNBLENDING_API int32_t unblending_hip_mlp_set_params(void* model, const float* params, void* stream);
+UNBLENDING_API int32_t unblending_hip_mlp_get_params(const void* model, float* params, void* stream);
+UNBLENDING_API size_t unblending_hip_mlp_train_workspace_size(int32_t batch);
+/* Adam steps on samples drawn from `count` images [count][height][width][3]
+ * with their solved first passes [count][layers][height*width][RGBA] as
+ * targets.  steps_done is the number of steps already taken (bias
+ * correction).  losses (host, may be NULL) receives one loss per step.
+ * fixed_samples (device, may be NULL): batch pixel indices for every step. */
+UNBLENDING_API int32_t unblending_hip_mlp_train(
+    void* model, const double* images, int32_t count, int32_t width, int32_t height,
+    const double* targets, int32_t layers, int32_t steps, int32_t steps_done, int32_t batch,
+    float learning_rate, uint64_t seed, const int64_t* fixed_samples, int32_t update,
+    void* workspace, size_t workspace_size, void* stream, float* losses, float* gradients);
+/* Predicted first pass for one image, packed [layers][height*width][RGBA]. */
+UNBLENDING_API size_t unblending_hip_mlp_predict_workspace_size(void);
+UNBLENDING_API int32_t unblending_hip_mlp_predict(
+    const void* model, const double* rgb, int32_t width, int32_t height, int32_t layers,
+    double* output, void* workspace, size_t workspace_size, void* stream);
 
 /* Planar seam, host pointers (see above).  Returns an unblending status. */
 UNBLENDING_API int32_t uhip_solve_planes(

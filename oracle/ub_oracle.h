@@ -122,11 +122,18 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
 /* Solves one pixel.  init: NULL for the default first-pass initialisation,
  * else the refinement initial colours (3L) with target_alphas (L).
  * bg_color: smoothed background colour (smooth refinement) or NULL.
- * x_out receives 4L values.  Returns 0. */
+ * x_out receives 4L values.  Returns 1 when the outer loop stopped at
+ * max_count without meeting both tolerances, else 0. */
 int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int opaque,
                     const double* init_colors, const double* target_alphas,
                     const double* bg_color, const orc_settings* settings, double* x_out,
                     orc_counters* counters);
+/* First pass started from x_init (4L values) and, when lambda_init is not
+ * NULL, from those 4 multipliers instead of the default initialisation
+ * (warm start); alpha_0 = 1 when opaque.  Returns as orc_solve_pixel. */
+int orc_solve_pixel_warm(const double target[3], int L, const orc_layer* layers, int opaque,
+                         const double* x_init, const double* lambda_init, const orc_settings* settings,
+                         double* x_out, orc_counters* counters);
 /* The inner minimiser alone (exposed for its invariant tests). */
 typedef double (*orc_fn)(const double* x, double* grad, void* data);
 int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn, void* data,

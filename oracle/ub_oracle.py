@@ -97,6 +97,9 @@ def lib():
         L.orc_solve_pixel.restype = ctypes.c_int
         L.orc_solve_pixel.argtypes = [dp, ctypes.c_int, lp, ctypes.c_int, dp, dp, dp,
                                       ctypes.POINTER(OrcSettings), dp, ctypes.POINTER(OrcCounters)]
+        L.orc_solve_pixel_warm.restype = ctypes.c_int
+        L.orc_solve_pixel_warm.argtypes = [dp, ctypes.c_int, lp, ctypes.c_int, dp, dp, ctypes.POINTER(OrcSettings),
+                                           dp, ctypes.POINTER(OrcCounters)]
         L.orc_minimize.restype = ctypes.c_int
         L.orc_minimize.argtypes = [dp, dp, dp, ctypes.c_int, ORC_FN, ctypes.c_void_p,
                                    ctypes.POINTER(OrcSettings), dp, ctypes.POINTER(ctypes.c_int32),
@@ -236,6 +239,20 @@ def solve_pixel(target, layers, opaque=True, init_colors=None, target_alphas=Non
                           None if ic is None else _dp(ic), None if ta is None else _dp(ta),
                           None if b is None else _dp(b), ctypes.byref(st), _dp(out), ctypes.byref(cnt))
     return out, cnt
+
+
+def solve_pixel_warm(target, layers, x_init, lambda_init=None, opaque=True, settings=None):
+    L = len(layers)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    xi = np.ascontiguousarray(x_init, dtype=np.float64)
+    li = None if lambda_init is None else np.ascontiguousarray(lambda_init, dtype=np.float64)
+    out = np.zeros(4 * L)
+    cnt = OrcCounters()
+    st = settings or default_settings()
+    capped = lib().orc_solve_pixel_warm(_dp(t), L, layer_array(layers), int(bool(opaque)), _dp(xi),
+                                        None if li is None else _dp(li), ctypes.byref(st), _dp(out),
+                                        ctypes.byref(cnt))
+    return out, cnt, bool(capped)
 
 
 def minimize(fn, x0, lb, ub, settings=None):

@@ -213,11 +213,38 @@ static double norm(const double* v, int n)
     return sqrt(s);
 }
 
+static int solve_pixel_from(const double target[3], int L, const orc_layer* layers, int opaque,
+                            const double* init_colors, const double* target_alphas,
+                            const double* bg_color, const double* x_init, const orc_settings* st,
+                            double* x_out, orc_counters* counters);
+static _Thread_local const double* g_lambda_init = NULL;
+
 /* unblending.cpp:155-309 */
 int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int opaque,
                     const double* init_colors, const double* target_alphas,
                     const double* bg_color, const orc_settings* st, double* x_out,
                     orc_counters* counters)
+{
+    return solve_pixel_from(target, L, layers, opaque, init_colors, target_alphas, bg_color, NULL, st,
+                            x_out, counters);
+}
+
+/* First pass started from x_init (4L values) instead of the default
+ * initialisation; the opaque background still sets alpha_0 = 1. */
+int orc_solve_pixel_warm(const double target[3], int L, const orc_layer* layers, int opaque,
+                         const double* x_init, const double* lambda_init, const orc_settings* st,
+                         double* x_out, orc_counters* counters)
+{
+    g_lambda_init = lambda_init;
+    const int r = solve_pixel_from(target, L, layers, opaque, NULL, NULL, NULL, x_init, st, x_out, counters);
+    g_lambda_init = NULL;
+    return r;
+}
+
+static int solve_pixel_from(const double target[3], int L, const orc_layer* layers, int opaque,
+                            const double* init_colors, const double* target_alphas,
+                            const double* bg_color, const double* x_init, const orc_settings* st,
+                            double* x_out, orc_counters* counters)
 {
     const int N = 4 * L;
     const int refine = target_alphas != NULL;
@@ -235,6 +262,8 @@ int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int 
         for (int c = 0; c < 3; ++c)
             x[L + 3 * i + c] = refine ? init_colors[3 * i + c] : layers[i].mean[c];
     }
+    if (x_init)
+        for (int i = 0; i < N; ++i) x[i] = x_init[i];
     if (opaque) { /* :194-198 */
         lb[0] = 1.0;
         if (refine && st->alpha_fixed) ub[0] = 1.0;
@@ -250,10 +279,10 @@ int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int 
     p.target = target;
     p.target_alphas = target_alphas;
     const int nc = refine ? 3 + L : 4;
-    for (int j = 0; j < nc; ++j) p.lambda[j] = 0.0;
+    for (int j = 0; j < nc; ++j) p.lambda[j] = g_lambda_init ? g_lambda_init[j] : 0.0;
     p.rho = st->initial_rho;
 
-    int count = 0;
+    int count = 0, capped = 0;
     for (;;) {
         memcpy(xn, x, sizeof(double) * (size_t)N);
         int32_t ev = 0, it = 0;
@@ -273,10 +302,14 @@ int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int 
         const int unchanged = norm(dx, N) < st->epsilon;
         const int satisfied = ngn < st->epsilon;
         memcpy(x, xn, sizeof(double) * (size_t)N);
-        if ((unchanged && satisfied) || count > st->max_count) break;
+        if (unchanged && satisfied) break;
+        if (count > st->max_count) {
+            capped = 1;
+            break;
+        }
         ++count;
     }
     memcpy(x_out, x, sizeof(double) * (size_t)N);
     if (counters) counters->pixels += 1;
-    return 0;
+    return capped;
 }

@@ -190,9 +190,10 @@ def test_environment_backend(monkeypatch):
 def test_device_entry_points_validate_workspace():
     lib = U.lib()
     L = layers(3)
-    assert lib.unblending_hip_workspace_size(4, 3, 3, 0, 0) >= 32
-    assert lib.unblending_hip_workspace_size(4, 3, 3, 1, 1) > lib.unblending_hip_workspace_size(4, 3, 3, 1, 0)
-    assert lib.unblending_hip_workspace_size(0, 3, 3, 1, 0) == 0
+    assert lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 0) >= 32
+    assert lib.unblending_hip_workspace_size(4, 3, 3, 1, 1, 0) > lib.unblending_hip_workspace_size(4, 3, 3, 1, 0, 0)
+    assert lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 1) > lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 0)
+    assert lib.unblending_hip_workspace_size(0, 3, 3, 1, 0, 0) == 0
     opt = U.options(U.BACKEND_HIP)
     # null rgb, then a too-small workspace: rejected before any device work
     assert lib.unblending_hip_unmix_device(None, 4, 3, L, 3, ctypes.byref(opt), None, ctypes.c_void_p(16), None,
@@ -200,3 +201,23 @@ def test_device_entry_points_validate_workspace():
     assert lib.unblending_hip_unmix_device(ctypes.c_void_p(16), 4, 3, L, 3, ctypes.byref(opt), None,
                                            ctypes.c_void_p(16), None, ctypes.c_void_p(16), 1, None,
                                            None) == U.BUFFER_TOO_SMALL
+
+
+def test_warm_start_model_interface_checks():
+    lib = U.lib()
+    assert lib.unblending_hip_mlp_param_count() == 41376
+    assert lib.unblending_hip_mlp_model_size() == 3 * 4 * 41376 + 2 * 2 * (4096 + 16384 + 16384 + 4096)
+    assert lib.unblending_hip_mlp_train_workspace_size(0) == 0
+    assert lib.unblending_hip_mlp_train_workspace_size(100) == 0
+    assert lib.unblending_hip_mlp_train_workspace_size(128) < lib.unblending_hip_mlp_train_workspace_size(256)
+    assert lib.unblending_hip_mlp_predict_workspace_size() > 0
+    buf = ctypes.c_void_p(16)  # never dereferenced: every call below fails its argument checks first
+    # more than 8 layers, bad batch, missing buffers
+    assert lib.unblending_hip_mlp_predict(buf, buf, 4, 4, 9, buf, buf, 1 << 30, None) == U.INVALID_ARGUMENT
+    assert lib.unblending_hip_mlp_predict(None, buf, 4, 4, 3, buf, buf, 1 << 30, None) == U.INVALID_ARGUMENT
+    assert lib.unblending_hip_mlp_predict(buf, buf, 4, 4, 3, buf, buf, 16, None) == U.BUFFER_TOO_SMALL
+    assert lib.unblending_hip_mlp_train(buf, buf, 1, 4, 4, buf, 3, 1, 0, 100, 1e-3, 0, None, 1, buf, 1 << 30,
+                                        None, None, None) == U.INVALID_ARGUMENT
+    assert lib.unblending_hip_mlp_train(buf, buf, 1, 4, 4, buf, 3, 1, 0, 128, 1e-3, 0, None, 1, buf, 16,
+                                        None, None, None) == U.BUFFER_TOO_SMALL
+    assert lib.unblending_hip_mlp_set_params(None, None, None) == U.INVALID_ARGUMENT

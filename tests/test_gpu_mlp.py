@@ -1,0 +1,258 @@
+"""GPU checks of the warm-start network and the warm-started first pass.
+
+The network runs BF16 matrix-core products with FP32 accumulation; its
+oracle (oracle/ub_mlp_oracle.py) rounds the same operands to BF16 and
+accumulates in FP64, so the bars are:
+  forward    max |y_gpu - y_oracle| <= 2e-3 on the sigmoid outputs
+  gradient   per weight matrix / bias, |g_gpu - g_oracle| / |g_oracle| <= 2e-2
+  loss       relative 1e-3
+  Adam       the FP32 update of the returned gradient within 1e-6 absolute
+  sampling   the batch the device draws equals the oracle's SplitMix64 batch
+             (checked through the gradient, which any other batch changes)
+The warm-started first pass is checked against the oracle's solver started
+from the same predicted points, pixel by pixel, at the first-pass bar
+(1e-3 max abs; the expected difference is zero), with the pixels the
+oracle's warm solve leaves at the outer-iteration cap taken from its plain
+solve, as the device's fallback launch does.
+"""
+import numpy as np
+import pytest
+
+import ub_mlp_oracle as M
+from conftest import layer_infos, oracle_layers
+
+pytestmark = pytest.mark.gpu
+
+FORWARD_TOL = 2e-3
+GRAD_REL_TOL = 2e-2
+LOSS_REL_TOL = 1e-3
+ADAM_TOL = 1e-6
+ALPHA_COLOR_TOL = 1e-3
+
+
+def _perturbed_params(L, seed):
+    from unblending_amd import mlp
+    rng = np.random.default_rng(seed)
+    p = mlp.initial_params(L, seed).astype(np.float64)
+    W, b = M.unpack(p)
+    for l in range(4):
+        b[l][:] = rng.normal(0, 0.1, b[l].size)
+    b[3][4 * L:] = 0.0
+    return M.pack(W, b).astype(np.float32)
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_forward_matches_oracle():
+    from unblending_amd import mlp
+    L = 5
+    params = _perturbed_params(L, 1)
+    model = mlp.WarmStartModel(L, params=params)
+    rng = np.random.default_rng(0)
+    rgb = rng.uniform(0, 1, (23, 37, 3))  # 851 pixels: a partial tile of 32 rows
+    print("case forward 37x23 L=5", flush=True)
+    y = model.predict(_dev(rgb)).cpu().numpy()
+    ref = M.predict(params, rgb, L)
+    d = np.abs(y - ref)
+    print("forward max|diff| %.3e mean %.3e" % (d.max(), d.mean()), flush=True)
+    assert d.max() <= FORWARD_TOL
+
+
+def test_forward_across_prediction_chunks():
+    """An image larger than one prediction chunk (262144 pixels): pixels on
+    both sides of the chunk boundary and at the end."""
+    from unblending_amd import mlp
+    L = 3
+    params = _perturbed_params(L, 2)
+    model = mlp.WarmStartModel(L, params=params)
+    w, h = 700, 400
+    rgb = np.random.default_rng(1).uniform(0, 1, (h, w, 3))
+    print("case forward 700x400 L=3", flush=True)
+    y = model.predict(_dev(rgb)).cpu().numpy().reshape(L, -1, 4)
+    idx = np.r_[0:64, 262144 - 64:262144 + 64, w * h - 64:w * h]
+    X = M.patches(rgb[None], idx, w, h)
+    _, _, yo = M.forward(params, X, bf16_ops=True)
+    ref = yo[:, :4 * L].reshape(len(idx), L, 4).transpose(1, 0, 2)
+    d = np.abs(y[:, idx, :] - ref)
+    print("chunked forward max|diff| %.3e" % d.max(), flush=True)
+    assert d.max() <= FORWARD_TOL
+
+
+def _train_problem(L=4, count=2, w=29, h=19, seed=0):
+    rng = np.random.default_rng(seed)
+    images = rng.uniform(0, 1, (count, h, w, 3))
+    targets = rng.uniform(0.05, 0.95, (count, L, h, w, 4))
+    return images, targets
+
+
+def _check_gradient(g, ref):
+    for l, (i, o) in enumerate(M.SHAPES):
+        for name, a, b in (("W%d" % (l + 1), g[M.OFF_W[l]:M.OFF_W[l] + i * o], ref[M.OFF_W[l]:M.OFF_W[l] + i * o]),
+                           ("b%d" % (l + 1), g[M.OFF_B[l]:M.OFF_B[l] + o], ref[M.OFF_B[l]:M.OFF_B[l] + o])):
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            print("  %s rel %.2e" % (name, rel), flush=True)
+            assert rel <= GRAD_REL_TOL, name
+
+
+def test_gradient_matches_oracle():
+    import torch
+    from unblending_amd import mlp
+    L, B = 4, 256
+    images, targets = _train_problem(L)
+    count, h, w, _ = images.shape
+    params = _perturbed_params(L, 3)
+    model = mlp.WarmStartModel(L, params=params)
+    idx = np.random.default_rng(4).integers(0, count * h * w, B)
+    print("case gradient fixed batch B=256 L=4", flush=True)
+    losses, g = model.train(_dev(images), _dev(targets), 1, batch=B, update=False, gradients=True,
+                            fixed_samples=torch.from_numpy(idx).cuda())
+    X = M.patches(images, idx, w, h)
+    T = M.targets_of(targets, idx, L, w, h, 4 * L)
+    Lref, gref = M.loss_grad(params, X, T, 4 * L, bf16_ops=True)
+    print("loss gpu %.6e oracle %.6e" % (losses[0], Lref), flush=True)
+    assert losses[0] == pytest.approx(Lref, rel=LOSS_REL_TOL)
+    _check_gradient(g.astype(np.float64), gref)
+    assert np.array_equal(model.params(), params)  # update=False leaves the weights
+
+
+def test_sampled_batch_matches_oracle_sampler():
+    from unblending_amd import mlp
+    L, B, seed = 3, 128, 0x1234
+    images, targets = _train_problem(L, count=3, w=17, h=13, seed=5)
+    count, h, w, _ = images.shape
+    params = _perturbed_params(L, 6)
+    model = mlp.WarmStartModel(L, params=params)
+    print("case sampled batch B=128", flush=True)
+    losses, g = model.train(_dev(images), _dev(targets), 1, batch=B, seed=seed, update=False, gradients=True)
+    idx = M.sample_indices(seed, 0, B, count * h * w)
+    X = M.patches(images, idx, w, h)
+    T = M.targets_of(targets, idx, L, w, h, 4 * L)
+    Lref, gref = M.loss_grad(params, X, T, 4 * L, bf16_ops=True)
+    assert losses[0] == pytest.approx(Lref, rel=LOSS_REL_TOL)
+    _check_gradient(g.astype(np.float64), gref)
+
+
+def test_adam_update_matches_oracle():
+    import torch
+    from unblending_amd import mlp
+    L, B, lr = 2, 64, 3e-3
+    images, targets = _train_problem(L, seed=7)
+    count, h, w, _ = images.shape
+    p0 = _perturbed_params(L, 8)
+    model = mlp.WarmStartModel(L, params=p0)
+    idx = torch.from_numpy(np.random.default_rng(9).integers(0, count * h * w, B)).cuda()
+    print("case adam two steps", flush=True)
+    m = np.zeros(p0.size, dtype=np.float32)
+    v = np.zeros(p0.size, dtype=np.float32)
+    p = p0.copy()
+    for step in range(2):
+        _, g = model.train(_dev(images), _dev(targets), 1, batch=B, lr=lr, fixed_samples=idx, gradients=True)
+        p, m, v = M.adam(p.astype(np.float32), m, v, g.astype(np.float32), step, np.float32(lr),
+                         np.float32(0.9), np.float32(0.999), np.float32(1e-8))
+        got = model.params()
+        d = np.abs(got - p)
+        print("  step %d max|diff| %.3e" % (step, d.max()), flush=True)
+        assert d.max() <= ADAM_TOL
+        p = got
+    pads = M.unpack(got)[0]
+    assert np.all(pads[0][M.INPUTS:, :] == 0.0) and np.all(pads[3][:, 4 * L:] == 0.0)
+
+
+def _training_set(k, size, seeds):
+    from unblending_amd import synth
+    return [synth.config(k, size, image_seed=s) for s in seeds]
+
+
+def _targets(wls):
+    import torch
+    import unblending_amd as U
+    from unblending_amd import device
+    h, w, _ = wls[0].rgb.shape
+    L = len(wls[0].layers)
+    images = torch.stack([_dev(wl.rgb) for wl in wls]).contiguous()
+    targets = torch.empty((len(wls), L, h, w, 4), dtype=torch.float64, device="cuda")
+    for j, wl in enumerate(wls):
+        device.unmix(images[j], layer_infos(wl.layers), out=targets[j])
+    return images, targets
+
+
+def test_training_reduces_loss():
+    from unblending_amd import mlp
+    wls = _training_set(1, (48, 40), [11, 12])
+    print("case training c1 48x40 x2", flush=True)
+    images, targets = _targets(wls)
+    model = mlp.WarmStartModel(3, seed=1)
+    losses = model.fit(images, targets, steps=600, batch=1024, lr=3e-3, chunk=200)
+    first, last = float(np.mean(losses[:20])), float(np.mean(losses[-50:]))
+    print("loss %.4e -> %.4e" % (first, last), flush=True)
+    assert np.all(np.isfinite(losses))
+    assert last < 0.3 * first
+
+
+@pytest.fixture(scope="module")
+def c1_model():
+    from unblending_amd import mlp
+    wls = _training_set(1, (64, 48), [21, 22, 23])
+    images, targets = _targets(wls)
+    model = mlp.WarmStartModel(3, seed=2)
+    model.fit(images, targets, steps=800, batch=2048, lr=3e-3, chunk=200)
+    return model
+
+
+def test_warm_unmix_matches_oracle(oracle, c1_model):
+    import unblending_amd as U
+    from unblending_amd import device, synth
+    wl = synth.config(1, (40, 30), image_seed=99)
+    L = len(wl.layers)
+    print("case warm unmix c1 40x30", flush=True)
+    rgb = _dev(wl.rgb)
+    pred = c1_model.predict(rgb).cpu().numpy()
+    stats = U.HipStats()
+    out = device.unmix(rgb, layer_infos(wl.layers), params=U.hip_params(warm_start=c1_model),
+                       stats=stats).cpu().numpy()
+    ol = oracle_layers(wl.layers)
+    h, w, _ = wl.rgb.shape
+    ref = np.empty_like(out)
+    capped = 0
+    for y in range(h):
+        for x in range(w):
+            x0 = np.concatenate([pred[:, y, x, 3], pred[:, y, x, :3].reshape(-1)])
+            sol, _, cap = oracle.solve_pixel_warm(wl.rgb[y, x], ol, x0)
+            if cap:
+                capped += 1
+                sol, _ = oracle.solve_pixel(wl.rgb[y, x], ol)
+            ref[:, y, x, :3] = sol[L:].reshape(L, 3)
+            ref[:, y, x, 3] = sol[:L]
+    d = np.abs(out - ref)
+    exact = float(np.mean(np.all(d == 0, axis=(0, 3))))
+    print("warm unmix max|diff| %.3e bit-identical pixels %.4f fallback %d (oracle %d)"
+          % (d.max(), exact, stats.fallback_pixels, capped), flush=True)
+    assert d.max() <= ALPHA_COLOR_TOL
+    assert stats.fallback_pixels == capped
+    assert stats.pixels == w * h + capped
+
+
+def test_warm_start_saves_evaluations(c1_model):
+    import unblending_amd as U
+    from unblending_amd import device, synth
+    wl = synth.config(1, (64, 48), image_seed=77)
+    rgb = _dev(wl.rgb)
+    infos = layer_infos(wl.layers)
+    print("case warm evals c1 64x48", flush=True)
+    s_plain, s_warm = U.HipStats(), U.HipStats()
+    plain = device.unmix(rgb, infos, stats=s_plain).cpu().numpy()
+    warm = device.unmix(rgb, infos, params=U.hip_params(warm_start=c1_model), stats=s_warm).cpu().numpy()
+    ratio = s_warm.evals / s_plain.evals
+    print("evals plain %d warm %d (ratio %.3f), fallback %d of %d pixels"
+          % (s_plain.evals, s_warm.evals, ratio, s_warm.fallback_pixels, plain.shape[1] * plain.shape[2]),
+          flush=True)
+    # both are points the solver accepts: the recomposites agree with the image alike
+    rp, _ = synth.composite(plain[..., 3], plain[..., :3], wl.layers)
+    rw, _ = synth.composite(warm[..., 3], warm[..., :3], wl.layers)
+    ep, ew = np.sqrt(np.mean((rp - wl.rgb) ** 2)), np.sqrt(np.mean((rw - wl.rgb) ** 2))
+    print("recomposite rms vs image: plain %.3e warm %.3e" % (ep, ew), flush=True)
+    assert ew <= 1.5 * ep + 1e-4
+    assert ratio < 0.9

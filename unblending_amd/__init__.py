@@ -149,7 +149,8 @@ class Options(ctypes.Structure):
 class HipParams(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("filter_radius", ctypes.c_int32),
                 ("filter_epsilon", ctypes.c_double), ("alpha_fixed", ctypes.c_int32),
-                ("solver", ctypes.c_int32), ("timing", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("solver", ctypes.c_int32), ("timing", ctypes.c_int32), ("warm_start", ctypes.c_int32),
+                ("mlp", ctypes.c_void_p)]
 
 
 class HipStats(ctypes.Structure):
@@ -157,7 +158,8 @@ class HipStats(ctypes.Structure):
                 ("pixels", ctypes.c_uint64), ("solve_ms", ctypes.c_double), ("filter_ms", ctypes.c_double),
                 ("filter_coeff_ms", ctypes.c_double), ("filter_out_ms", ctypes.c_double),
                 ("filter_radius", ctypes.c_int32), ("filter_channels", ctypes.c_int32),
-                ("filter_launches", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("filter_launches", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("fallback_pixels", ctypes.c_uint64), ("predict_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -183,7 +185,7 @@ _lib.unblending_hip_refine_ex.argtypes = [ctypes.POINTER(RgbView), ctypes.POINTE
                                           ctypes.POINTER(HipStats)]
 _lib.unblending_hip_workspace_size.restype = ctypes.c_size_t
 _lib.unblending_hip_workspace_size.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_size_t,
-                                               ctypes.c_int32, ctypes.c_int32]
+                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
 _lib.unblending_hip_unmix_device.restype = ctypes.c_int32
 _lib.unblending_hip_unmix_device.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CLayer),
                                              ctypes.c_size_t, ctypes.POINTER(Options),
@@ -204,6 +206,27 @@ _lib.unblending_hip_guided_filter_device.argtypes = [_vp, ctypes.c_int32, ctypes
 _lib.uhip_solve_planes.restype = ctypes.c_int32
 _lib.uhip_solve_planes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CLayer), ctypes.c_size_t,
                                    _dp, _dp, ctypes.c_int32, ctypes.c_int32, _dp]
+_lib.unblending_hip_mlp_model_size.restype = ctypes.c_size_t
+_lib.unblending_hip_mlp_model_size.argtypes = []
+_lib.unblending_hip_mlp_param_count.restype = ctypes.c_int32
+_lib.unblending_hip_mlp_param_count.argtypes = []
+_lib.unblending_hip_mlp_set_params.restype = ctypes.c_int32
+_lib.unblending_hip_mlp_set_params.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), _vp]
+_lib.unblending_hip_mlp_get_params.restype = ctypes.c_int32
+_lib.unblending_hip_mlp_get_params.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), _vp]
+_lib.unblending_hip_mlp_train_workspace_size.restype = ctypes.c_size_t
+_lib.unblending_hip_mlp_train_workspace_size.argtypes = [ctypes.c_int32]
+_lib.unblending_hip_mlp_train.restype = ctypes.c_int32
+_lib.unblending_hip_mlp_train.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_float, ctypes.c_uint64, _vp, ctypes.c_int32, _vp,
+                                          ctypes.c_size_t, _vp, ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(ctypes.c_float)]
+_lib.unblending_hip_mlp_predict_workspace_size.restype = ctypes.c_size_t
+_lib.unblending_hip_mlp_predict_workspace_size.argtypes = []
+_lib.unblending_hip_mlp_predict.restype = ctypes.c_int32
+_lib.unblending_hip_mlp_predict.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
+                                            ctypes.c_size_t, _vp]
 _lib.unblending_hip_last_error.restype = ctypes.c_char_p
 
 
@@ -245,7 +268,8 @@ def options(backend=BACKEND_AUTO, opaque=True, smooth=False, target_concurrency=
     return o
 
 
-def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False) -> HipParams:
+def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, warm_start=None) -> HipParams:
+    """warm_start: a trained ``mlp.WarmStartModel`` (first pass only), or None."""
     p = HipParams()
     p.struct_size = ctypes.sizeof(HipParams)
     p.filter_radius = int(radius)
@@ -253,6 +277,8 @@ def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False) ->
     p.alpha_fixed = int(bool(alpha_fixed))
     p.solver = int(solver)
     p.timing = int(bool(timing))
+    p.warm_start = int(warm_start is not None)
+    p.mlp = warm_start.data_ptr() if warm_start is not None else None
     return p
 
 

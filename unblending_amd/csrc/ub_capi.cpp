@@ -228,7 +228,7 @@ int32_t device_call(const double* rgb, int32_t width, int32_t height, const unbl
             if (!set.all_plus && !set.all_so) return UNBLENDING_INVALID_ARGUMENT;
         }
         const size_t need = unblending_hip_workspace_size(width, height, layer_count, first != nullptr,
-                                                          smooth && first != nullptr);
+                                                          smooth && first != nullptr, p.warm_start);
         if (!workspace || workspace_size < need) return UNBLENDING_BUFFER_TOO_SMALL;
         DeviceJob job;
         job.w = width;
@@ -308,10 +308,11 @@ UNBLENDING_API int32_t unblending_hip_refine_ex(const unblending_rgb_view* image
 }
 
 UNBLENDING_API size_t unblending_hip_workspace_size(int32_t width, int32_t height, size_t layer_count,
-                                                   int32_t refine, int32_t smooth_background)
+                                                   int32_t refine, int32_t smooth_background, int32_t warm_start)
 {
     if (width <= 0 || height <= 0 || layer_count == 0) return 0;
-    return workspace_bytes(size_t(width) * size_t(height), layer_count, refine != 0, smooth_background != 0);
+    return workspace_bytes(size_t(width) * size_t(height), layer_count, refine != 0, smooth_background != 0,
+                           warm_start != 0);
 }
 
 UNBLENDING_API int32_t unblending_hip_unmix_device(const double* rgb, int32_t width, int32_t height,
@@ -411,6 +412,116 @@ UNBLENDING_API int32_t uhip_solve_planes(int32_t width, int32_t height, const un
     } catch (const std::bad_alloc&) {
         return UNBLENDING_OUT_OF_MEMORY;
     }
+}
+
+UNBLENDING_API size_t unblending_hip_mlp_model_size(void) { return mlp::model_bytes(); }
+
+UNBLENDING_API int32_t unblending_hip_mlp_param_count(void) { return mlp::kParamsPadded; }
+
+UNBLENDING_API int32_t unblending_hip_mlp_set_params(void* model, const float* params, void* stream)
+{
+    g_last_error.clear();
+    if (!model || !params) return UNBLENDING_INVALID_ARGUMENT;
+    const mlp::Model md = mlp::model_view(model);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(md.w, params, sizeof(float) * mlp::kParamsPadded, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(md.m, 0, 2 * sizeof(float) * mlp::kParamsPadded, s);
+    if (e == hipSuccess) e = mlp_refresh(md, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        g_last_error = hipGetErrorString(e);
+        return UNBLENDING_DEVICE_ERROR;
+    }
+    return UNBLENDING_OK;
+}
+
+UNBLENDING_API int32_t unblending_hip_mlp_get_params(const void* model, float* params, void* stream)
+{
+    g_last_error.clear();
+    if (!model || !params) return UNBLENDING_INVALID_ARGUMENT;
+    const mlp::Model md = mlp::model_view(const_cast<void*>(model));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(params, md.w, sizeof(float) * mlp::kParamsPadded, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        g_last_error = hipGetErrorString(e);
+        return UNBLENDING_DEVICE_ERROR;
+    }
+    return UNBLENDING_OK;
+}
+
+UNBLENDING_API size_t unblending_hip_mlp_train_workspace_size(int32_t batch)
+{
+    if (batch <= 0 || batch % 64) return 0;
+    return mlp_train_bytes(batch);
+}
+
+UNBLENDING_API int32_t unblending_hip_mlp_train(void* model, const double* images, int32_t count, int32_t width,
+                                                int32_t height, const double* targets, int32_t layers,
+                                                int32_t steps, int32_t steps_done, int32_t batch,
+                                                float learning_rate, uint64_t seed, const int64_t* fixed_samples,
+                                                int32_t update, void* workspace, size_t workspace_size, void* stream,
+                                                float* losses, float* gradients)
+{
+    g_last_error.clear();
+    if (!model || !images || !targets || count <= 0 || width <= 0 || height <= 0 || layers <= 0 ||
+        4 * layers > mlp::kOutPad || steps < 0 || steps_done < 0 || batch <= 0 || batch % 64)
+        return UNBLENDING_INVALID_ARGUMENT;
+    if (!workspace || workspace_size < mlp_train_bytes(batch)) return UNBLENDING_BUFFER_TOO_SMALL;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const mlp::Model md = mlp::model_view(model);
+    const mlp::TrainBuffers buf = mlp_train_view(workspace, batch);
+    mlp::TrainBatch tb;
+    tb.images = images;
+    tb.targets = targets;
+    tb.w = width;
+    tb.h = height;
+    tb.layers = layers;
+    tb.outputs = 4 * layers;
+    tb.batch = batch;
+    tb.pool = (long long)count * width * height;
+    tb.seed = seed;
+    tb.fixed_idx = reinterpret_cast<const long long*>(fixed_samples);
+    hipError_t e = hipSuccess;
+    for (int step = 0; step < steps && e == hipSuccess; ++step) {
+        const int slot = step % mlp::kLossSlots;
+        e = mlp_step(md, tb, buf, steps_done + step, update != 0, learning_rate, buf.loss + slot, s);
+        // per-step losses come back a block of slots at a time
+        if (e == hipSuccess && losses && (slot == mlp::kLossSlots - 1 || step == steps - 1))
+            e = hipMemcpyAsync(losses + (step - slot), buf.loss, sizeof(float) * (slot + 1), hipMemcpyDeviceToHost,
+                               s);
+        if (e == hipSuccess && losses && slot == mlp::kLossSlots - 1) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess) {
+        g_last_error = hipGetErrorString(e);
+        return UNBLENDING_DEVICE_ERROR;
+    }
+    if (gradients) e = hipMemcpyAsync(gradients, buf.grad, sizeof(float) * mlp::kParamsPadded, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        g_last_error = hipGetErrorString(e);
+        return UNBLENDING_DEVICE_ERROR;
+    }
+    return UNBLENDING_OK;
+}
+
+UNBLENDING_API size_t unblending_hip_mlp_predict_workspace_size(void) { return mlp_predict_bytes(); }
+
+UNBLENDING_API int32_t unblending_hip_mlp_predict(const void* model, const double* rgb, int32_t width,
+                                                  int32_t height, int32_t layers, double* output, void* workspace,
+                                                  size_t workspace_size, void* stream)
+{
+    g_last_error.clear();
+    if (!model || !rgb || !output || width <= 0 || height <= 0 || layers <= 0 || 4 * layers > mlp::kOutPad)
+        return UNBLENDING_INVALID_ARGUMENT;
+    if (!workspace || workspace_size < mlp_predict_bytes()) return UNBLENDING_BUFFER_TOO_SMALL;
+    const hipError_t e = mlp_predict(mlp::model_view(const_cast<void*>(model)), rgb, width, height, 4 * layers,
+                                     output, mlp_predict_view(workspace), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        g_last_error = hipGetErrorString(e);
+        return UNBLENDING_DEVICE_ERROR;
+    }
+    return UNBLENDING_OK;
 }
 
 UNBLENDING_API const char* unblending_hip_last_error(void) { return g_last_error.c_str(); }

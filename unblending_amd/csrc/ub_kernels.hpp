@@ -26,8 +26,12 @@ struct SolverParams {
 };
 
 // Per-pass work counters accumulated by the solve kernels.
-// kQueue is the persistent solve's next-pixel counter.
-enum Counter : int { kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kNumCounters = 5 };
+// kQueue / kQueue2 are the persistent solve's next-item counters (first
+// launch, fallback launch); kFallback counts the pixels a warm-started solve
+// hands to the fallback launch.
+enum Counter : int {
+    kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6, kNumCounters = 8
+};
 
 struct PixelArgs {
     DevLayer layers[kMaxLayers];  // passed by value; staged into LDS by the kernel
@@ -42,6 +46,11 @@ struct PixelArgs {
     double* alpha_planes;  // optional copy of the solved alphas [L][n]
     double* lane_state;    // persistent solve: kLaneStateDoubles of per-lane scratch
     unsigned long long* counters;
+    // first pass only: warm start and fallback
+    const double* warm;    // predicted first pass, packed [L][n][RGBA], or nullptr
+    long long* fallback;   // warm start: pixels to re-solve from the default initialisation
+    const long long* list; // pixel list to solve instead of 0..n-1 (its length in counters[kFallback])
+    int queue;             // counter slot of the next-item queue
 };
 
 hipError_t launch_solve(int L, bool refine, const PixelArgs& args, hipStream_t stream);

@@ -119,9 +119,70 @@ size_t filter_workspace_bytes(size_t n, int channels)
     return align256(4 * group * n * sizeof(double));
 }
 
-size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth)
+namespace {
+constexpr long long kPredictChunk = 1 << 18;  // pixels per prediction chunk
+
+// Carves `bytes` (256-aligned) from a workspace cursor.
+template <class T>
+T* carve(char*& cur, size_t count)
+{
+    T* p = reinterpret_cast<T*>(cur);
+    cur += align256(count * sizeof(T));
+    return p;
+}
+}  // namespace
+
+namespace {
+// rows of one prediction chunk for an n-pixel image (a multiple of 32)
+long long predict_chunk(size_t n) { return std::min<long long>(kPredictChunk, (long long)((n + 31) / 32 * 32)); }
+}  // namespace
+
+size_t mlp_predict_bytes(size_t n) { return 3 * align256(size_t(predict_chunk(n)) * mlp::kHidden * 2); }
+
+mlp::PredictBuffers mlp_predict_view(void* ws, size_t n)
+{
+    mlp::PredictBuffers b;
+    char* cur = static_cast<char*>(ws);
+    b.chunk = predict_chunk(n);
+    for (int l = 0; l < 3; ++l) b.h[l] = carve<__bf16>(cur, size_t(b.chunk) * mlp::kHidden);
+    return b;
+}
+
+mlp::TrainBuffers mlp_train_view(void* ws, int batch)
+{
+    using namespace mlp;
+    mlp::TrainBuffers b;
+    char* cur = static_cast<char*>(ws);
+    const size_t B = size_t(batch);
+    b.idx = carve<long long>(cur, B);
+    for (int l = 0; l < 3; ++l) {
+        b.z[l] = carve<float>(cur, B * kHidden);
+        b.h[l] = carve<__bf16>(cur, B * kHidden);
+        b.ht[l] = carve<__bf16>(cur, B * kHidden);
+    }
+    for (int l = 0; l < 4; ++l) {
+        const size_t N = (l == 3) ? kOutPad : kHidden;
+        b.dz[l] = carve<float>(cur, B * N);
+        b.dzb[l] = carve<__bf16>(cur, B * N);
+        b.dzt[l] = carve<__bf16>(cur, B * N);
+    }
+    b.xt = carve<__bf16>(cur, B * kInPad);
+    b.grad = carve<float>(cur, kParamsPadded);
+    b.loss = carve<float>(cur, mlp::kLossSlots);
+    return b;
+}
+
+size_t mlp_train_bytes(int batch)
+{
+    // a dry run of the carving over a null base gives the size
+    const mlp::TrainBuffers b = mlp_train_view(nullptr, batch);
+    return size_t(reinterpret_cast<char*>(b.loss) - static_cast<char*>(nullptr)) + align256(mlp::kLossSlots * sizeof(float));
+}
+
+size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool warm)
 {
     size_t bytes = kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
+    if (warm && !refine) bytes += mlp_predict_bytes(n) + align256(n * sizeof(long long));
     if (refine) {
         const int C = filter_channels(layers, smooth);
         bytes += filter_workspace_bytes(n, C);
@@ -201,6 +262,25 @@ void run_device(const DeviceJob& job)
     check(launch_zero(counters, kNumCounters, job.stream), "zero counters");
 
     if (job.stats) *job.stats = unblending_hip_stats{};
+    const bool warm = !refine && job.params.warm_start != 0;
+    double predict_ms = 0.0;
+    if (warm) {
+        if (!job.params.mlp) throw std::invalid_argument("A warm start needs a model");
+        if (4 * L > size_t(mlp::kOutPad)) throw ExecutionError("The warm-start model predicts at most 8 layers");
+        char* wcur = ws + kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
+        const mlp::PredictBuffers pb = mlp_predict_view(wcur, n);
+        args.fallback = reinterpret_cast<long long*>(wcur + mlp_predict_bytes(n));
+        const mlp::Model md = mlp::model_view(const_cast<void*>(job.params.mlp));
+        Events ep(timing);
+        ep.start(job.stream);
+        // the prediction is written into the output buffer, which the solve
+        // reads pixel by pixel before writing the same pixel's solution
+        check(mlp_predict(md, job.rgb, job.w, job.h, int(4 * L), job.out, pb, job.stream), "warm-start prediction");
+        ep.stop(job.stream);
+        predict_ms = ep.ms();
+        args.warm = job.out;
+    }
+    args.queue = kQueue;
     if (refine) {
         int radius = job.params.filter_radius;
         if (radius < 0) radius = 60 * std::min(job.w, job.h) / 1000;  // unblending.cpp:351
@@ -222,10 +302,21 @@ void run_device(const DeviceJob& job)
 
     Events es(timing);
     es.start(job.stream);
+    if (warm && args.solver != 0) throw std::invalid_argument("The warm start runs on the persistent solve kernel");
     check(launch_solve(int(L), refine, args, job.stream), "solve");
+    if (warm) {
+        // re-solve, from the default initialisation, the pixels whose
+        // warm-started solve missed the tolerances
+        PixelArgs fb = args;
+        fb.warm = nullptr;
+        fb.list = args.fallback;
+        fb.fallback = nullptr;
+        fb.queue = kQueue2;
+        check(launch_solve(int(L), false, fb, job.stream), "fallback solve");
+    }
     es.stop(job.stream);
     if (job.stats) {
-        unsigned long long host_counters[kNumCounters] = {0, 0, 0, 0, 0};
+        unsigned long long host_counters[kNumCounters] = {};
         check(hipMemcpyAsync(host_counters, counters, sizeof host_counters, hipMemcpyDeviceToHost, job.stream),
               "counters");
         check(hipStreamSynchronize(job.stream), "hipStreamSynchronize");
@@ -233,7 +324,9 @@ void run_device(const DeviceJob& job)
         job.stats->iterations = host_counters[kIterations];
         job.stats->outer = host_counters[kOuter];
         job.stats->pixels = host_counters[kPixels];
+        job.stats->fallback_pixels = host_counters[kFallback];
         job.stats->solve_ms = es.ms();
+        job.stats->predict_ms = predict_ms;
     }
 }
 
@@ -248,7 +341,8 @@ void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const doubl
     DeviceBuffer d_rgb(3 * n * sizeof(double));
     DeviceBuffer d_out(4 * L * n * sizeof(double));
     DeviceBuffer d_first(first ? 4 * L * n * sizeof(double) : 0);
-    const size_t ws_bytes = workspace_bytes(n, L, first != nullptr, smooth);
+    const bool warm = first == nullptr && params.warm_start != 0;
+    const size_t ws_bytes = workspace_bytes(n, L, first != nullptr, smooth, warm);
     DeviceBuffer d_ws(ws_bytes);
     check(hipMemcpy2DAsync(d_rgb.p, 3 * w * sizeof(double), rgb.data, rgb.row_stride * sizeof(double),
                            3 * w * sizeof(double), h, hipMemcpyHostToDevice, stream.s),

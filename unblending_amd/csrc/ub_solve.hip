@@ -324,6 +324,16 @@ __device__ __forceinline__ void load_pixel(const PixelArgs& args, const DevLayer
 #pragma unroll
             for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = px[c];
         }
+    } else if (args.warm) {
+        // warm start: the predicted first pass
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            ta[l] = 0.0;
+            const double* px = args.warm + 4 * ((long long)l * args.n + i);
+            x[l] = px[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = px[c];
+        }
     } else {
 #pragma unroll
         for (int l = 0; l < L; ++l) {
@@ -502,6 +512,9 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         }
     };
 
+    // items: pixels 0..n-1, or the entries of a pixel list
+    const long long items = args.list ? (long long)args.counters[kFallback] : args.n;
+
     for (;;) {
         // The layer table is uniform: it is read with scalar loads from the
         // argument segment.  An opaque zero offset keeps those loads inside
@@ -516,13 +529,14 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         if (needy) {
             const int leader = __ffsll((long long)needy) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&args.counters[kQueue], (unsigned long long)__popcll(needy));
+            if (lane == leader) base = atomicAdd(&args.counters[args.queue], (unsigned long long)__popcll(needy));
             base = __shfl(base, leader);
             if (phase == kNeedPixel) {
-                pix = (long long)(base + (unsigned long long)__popcll(needy & ((1ull << lane) - 1ull)));
-                if (pix >= args.n) {
+                const long long item = (long long)(base + (unsigned long long)__popcll(needy & ((1ull << lane) - 1ull)));
+                if (item >= items) {
                     phase = kIdle;
                 } else {
+                    pix = args.list ? args.list[item] : item;
                     double x0[N], C0[NC];
                     load_pixel<L, REFINE>(args, lay, pix, tgt, ta, bg, x0);
 #pragma unroll
@@ -719,6 +733,13 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             const bool satisfied = ngn < sp.eps;
             cn_outer = ngn;
             if ((unchanged && satisfied) || ocount > sp.max_count) {
+                // warm start: a pixel whose solve stopped at the outer-iteration
+                // cap without meeting the plain solve's tolerances (residual
+                // |C| < eps and step < eps) goes to the fallback launch
+                if (!REFINE && args.fallback && !(unchanged && satisfied)) {
+                    const unsigned long long k = atomicAdd(&args.counters[kFallback], 1ull);
+                    args.fallback[k] = pix;
+                }
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
                     double* px = args.out + 4 * ((long long)l * args.n + pix);

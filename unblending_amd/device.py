@@ -34,13 +34,14 @@ def _stream_ptr(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def workspace_size(w, h, L, refine, smooth=False) -> int:
-    return int(_lib.unblending_hip_workspace_size(int(w), int(h), int(L), int(bool(refine)), int(bool(smooth))))
+def workspace_size(w, h, L, refine, smooth=False, warm=False) -> int:
+    return int(_lib.unblending_hip_workspace_size(int(w), int(h), int(L), int(bool(refine)), int(bool(smooth)),
+                                                  int(bool(warm))))
 
 
-def workspace(w, h, L, refine, smooth=False, device=None):
+def workspace(w, h, L, refine, smooth=False, device=None, warm=False):
     torch = _torch()
-    n = max(1, (workspace_size(w, h, L, refine, smooth) + 7) // 8)
+    n = max(1, (workspace_size(w, h, L, refine, smooth, warm) + 7) // 8)
     return torch.empty(n, dtype=torch.float64, device=device or "cuda")
 
 
@@ -56,7 +57,7 @@ def unmix(rgb, layer_infos, opaque=True, out=None, alpha_planes=None, ws=None, p
     if alpha_planes is not None:
         _check(alpha_planes, (L, h, w), "alpha_planes")
     if ws is None:
-        ws = workspace(w, h, L, False, device=rgb.device)
+        ws = workspace(w, h, L, False, device=rgb.device, warm=bool(params is not None and params.warm_start))
     opt = options(BACKEND_HIP, opaque, False)
     st = _lib.unblending_hip_unmix_device(
         ctypes.c_void_p(rgb.data_ptr()), w, h, layer_array(layer_infos), L, ctypes.byref(opt),

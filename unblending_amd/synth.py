@@ -20,7 +20,9 @@ configurations of the unmixing + refinement path:
 
 The bottom layer is Normal and opaque; every comp-op is source-over.  The
 seed of config k is 0x5EED0000 + k (image j of config 5: 0x5EED0500 + j).
-`size` shrinks an image for tests while keeping the recipe.
+`size` shrinks an image for tests while keeping the recipe; `image_seed`
+keeps config k's layer set and draws another image of it (the alpha fields
+and noise come from that seed).
 """
 from __future__ import annotations
 
@@ -188,21 +190,27 @@ def _render(rng, layers, alpha_fields, noise, h, w):
     return np.ascontiguousarray(np.clip(rgb, 0.0, 1.0))
 
 
-def config1(size=(256, 256), seed=0x5EED0001):
+def _image_rng(rng, image_seed):
+    return rng if image_seed is None else np.random.default_rng(image_seed)
+
+
+def config1(size=(256, 256), seed=0x5EED0001, image_seed=None):
     rng = np.random.default_rng(seed)
     w, h = size
     modes = [NORMAL, MULTIPLY, SCREEN]
     layers = [iso_model(rng, m, 0.01) for m in modes]
+    rng = _image_rng(rng, image_seed)
     fields = [None] + [blurred_noise(rng, h, w, 8.0) for _ in modes[1:]]
     rgb = _render(rng, layers, fields, 0.005, h, w)
     return Workload("c1 %dx%d L=3" % (w, h), rgb, layers, 8, 1e-4)
 
 
-def config2(size=(1024, 1024), seed=0x5EED0002):
+def config2(size=(1024, 1024), seed=0x5EED0002, image_seed=None):
     rng = np.random.default_rng(seed)
     w, h = size
     modes = [NORMAL, MULTIPLY, SCREEN, COLOR_DODGE, OVERLAY]
     layers = [spd_model(rng, m, 0.1, 0.9, 0.002, 0.02) for m in modes]
+    rng = _image_rng(rng, image_seed)
     fields = [None]
     for _ in modes[1:]:
         a = blurred_noise(rng, h, w, 16.0)
@@ -212,11 +220,12 @@ def config2(size=(1024, 1024), seed=0x5EED0002):
     return Workload("c2 %dx%d L=5" % (w, h), rgb, layers, 16, 1e-4)
 
 
-def config3(size=(4096, 4096), seed=0x5EED0003):
+def config3(size=(4096, 4096), seed=0x5EED0003, image_seed=None):
     rng = np.random.default_rng(seed)
     w, h = size
     modes = [NORMAL, MULTIPLY, SCREEN, OVERLAY, COLOR_BURN, HARD_LIGHT, SOFT_LIGHT]
     layers = [spd_model(rng, m, 0.05, 0.95, 0.001, 0.03) for m in modes]
+    rng = _image_rng(rng, image_seed)
     scale = min(w, h) / 4096.0
     fields = [None] + [blobs(rng, h, w, 20, 20.0 * max(scale, 0.05), 200.0 * max(scale, 0.05))
                        for _ in modes[1:]]
@@ -243,11 +252,11 @@ def config5_image(j, scale=1.0):
     return Workload("c5[%d] %dx%d L=%d" % (j, w, h, L), rgb, layers, r, 1e-4)
 
 
-def config(k, size=None):
+def config(k, size=None, image_seed=None):
     if k == 1:
-        return config1(size or (256, 256))
+        return config1(size or (256, 256), image_seed=image_seed)
     if k == 2:
-        return config2(size or (1024, 1024))
+        return config2(size or (1024, 1024), image_seed=image_seed)
     if k == 3:
-        return config3(size or (4096, 4096))
+        return config3(size or (4096, 4096), image_seed=image_seed)
     raise ValueError("config %r" % k)
