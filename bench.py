@@ -3,9 +3,16 @@
 A step is one unmix + refine of one synthetic image resident in HBM (the
 default workload: 4096 x 4096, 7 layers -- Normal, Multiply, Screen,
 Overlay, ColorBurn, HardLight, SoftLight -- guided filter r = 32,
-eps = 1e-3).  Prints one JSON line with Mpixel/s, the roofline of the
-dominant kernel (the first-pass solve, FP64), the filter's HBM rate, and the
-oracle timed on the host cores on a band of the same image.
+eps = 1e-3).  The step is timed twice: plain (every pixel starts at the
+reference's initial point) and with the warm-start network, trained first on
+2^20 pixels of other images of the same layer set solved by the plain GPU
+path (Adam, batch 4096, 20k steps; the training time is reported, not
+counted in the step).  Prints one JSON line: `value` is the warm-started
+Mpixel/s, with the plain rate, the difference between the two outputs, the
+roofline of the dominant kernel (FP64 solve), the filter's HBM rate, the
+network's matrix-core rate, and the oracle timed on the host cores on a band
+of the same image (whose first pass and refinement both runs are compared
+with).
 """
 from __future__ import annotations
 
@@ -22,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def solve_flops(evals: int, iterations: int, L: int) -> float:
@@ -38,6 +46,15 @@ def filter_bytes(n: int, channels: int) -> float:
     return 8.0 * (6 + 10 * channels) * n
 
 
+def mlp_train_flops(batch: int) -> float:
+    """Matrix-core operations of one training step: forward
+    2 B (32*128 + 2*128*128 + 128*32), the same again for the weight
+    gradients, and 2 B (32*128 + 2*128*128) for the input gradients of the
+    upper three layers."""
+    fwd = 32 * 128 + 2 * 128 * 128 + 128 * 32
+    return 2.0 * batch * (2 * fwd + 32 * 128 + 2 * 128 * 128)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -49,12 +66,14 @@ def cpu_baseline(wl, rows, threads):
     layers = [O.Layer(l.mode, tuple(l.mean), tuple(np.ravel(l.precision)), l.comp) for l in wl.layers]
     t0 = time.perf_counter()
     first, _ = O.unmix(band, layers, opaque=True, threads=threads)
-    O.refine(band, layers, first, opaque=True, radius=wl.radius, eps=wl.eps, threads=threads)
+    second, _ = O.refine(band, layers, first, opaque=True, radius=wl.radius, eps=wl.eps, threads=threads)
     dt = time.perf_counter() - t0
     n = band.shape[0] * band.shape[1]
-    return {"value": n / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
-            "sample": "oracle (FP64 C restatement), unmix + refine of rows 0-%d of the same image "
-                      "(%d x %d px), %d threads, %.1f s" % (rows - 1, band.shape[1], rows, threads, dt)}
+    rec = {"value": n / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
+           "target_concurrency": threads,
+           "sample": "oracle (FP64 C restatement), unmix + refine of rows 0-%d of the same image "
+                     "(%d x %d px), %d threads, %.1f s" % (rows - 1, band.shape[1], rows, threads, dt)}
+    return rec, first, second
 
 
 def main():
@@ -64,9 +83,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--size", type=str, default="")
-    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--cpu-rows", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solver", type=int, default=0)
+    ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--train-size", type=int, default=512)
+    ap.add_argument("--train-images", type=int, default=4)
+    ap.add_argument("--train-lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
@@ -95,33 +119,91 @@ def main():
     ws2 = device.workspace(w, h, L, True, device=dev)
     params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver)
 
-    def step(p=params, s1=None, s2=None):
+    def step(p=params, s1=None, s2=None, o=out):
         device.unmix(rgb, infos, True, out=first, alpha_planes=alpha, ws=ws1, params=p, stats=s1)
-        device.refine(rgb, first, infos, True, False, alpha_planes=alpha, out=out, ws=ws2, params=p, stats=s2)
+        device.refine(rgb, first, infos, True, False, alpha_planes=alpha, out=o, ws=ws2, params=p, stats=s2)
 
-    for i in range(args.warmup):
-        step()
+    def timed(p, o, label):
+        for i in range(args.warmup):
+            step(p, o=o)
+            torch.cuda.synchronize()
+            log("%s warmup step %d done" % (label, i))
         torch.cuda.synchronize()
-        log("warmup step %d done" % i)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / args.steps
-    value = n / (ms * 1e-3) / 1e6
-    log("timed %d steps: %.2f ms/step" % (args.steps, ms))
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step(p, o=o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        log("%s: timed %d steps, %.2f ms/step" % (label, args.steps, ms))
+        return ms
 
-    # one instrumented step: HIP events around each kernel on the launch stream
-    tparams = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, timing=True)
-    s1, s2 = U.HipStats(), U.HipStats()
-    step(tparams, s1, s2)
-    torch.cuda.synchronize()
+    def instrumented(p, o):
+        """One step with HIP events around each kernel on the launch stream."""
+        p.timing = 1
+        s1, s2 = U.HipStats(), U.HipStats()
+        step(p, s1, s2, o=o)
+        torch.cuda.synchronize()
+        p.timing = 0
+        return s1, s2
+
+    ms_plain = timed(params, out, "plain")
+    s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver), out)
+    first_plain = first.clone()
+
+    warm = None
+    if not args.no_warm:
+        from unblending_amd import mlp
+        ts = args.train_size
+        t0 = time.perf_counter()
+        tw = [synth.config(args.config, (ts, ts), image_seed=0x7EA1 + j) for j in range(args.train_images)]
+        log("generated %d training images %dx%d in %.1f s" % (len(tw), ts, ts, time.perf_counter() - t0))
+        images = torch.from_numpy(np.stack([t.rgb for t in tw])).to(dev).contiguous()
+        targets = torch.empty((len(tw), L, ts, ts, 4), dtype=torch.float64, device=dev)
+        wst = device.workspace(ts, ts, L, False, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j in range(len(tw)):
+            device.unmix(images[j], infos, True, out=targets[j], ws=wst)
+        torch.cuda.synchronize()
+        label_s = time.perf_counter() - t0
+        model = mlp.WarmStartModel(L, device=dev, seed=0)
+        t0 = time.perf_counter()
+        losses = model.fit(images, targets, steps=args.train_steps, batch=4096, lr=args.train_lr,
+                           final_lr=0.05 * args.train_lr, chunk=1000)
+        torch.cuda.synchronize()
+        train_s = time.perf_counter() - t0
+        log("trained %d steps in %.1f s, loss %.3e -> %.3e"
+            % (args.train_steps, train_s, float(np.mean(losses[:10])), float(np.mean(losses[-100:]))))
+        del images, targets, wst
+        wparams = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, warm_start=model)
+        ws1 = device.workspace(w, h, L, False, device=dev, warm=True)
+        out_warm = torch.empty_like(out)
+        ms_warm = timed(wparams, out_warm, "warm")
+        w1, w2 = instrumented(wparams, out_warm)
+        d1 = (first - first_plain).abs()
+        d2 = (out_warm - out).abs()
+        per_train = train_s / max(args.train_steps, 1)
+        warm = {"ms_per_step": ms_warm, "s1": w1, "s2": w2, "train_s": train_s, "label_s": label_s,
+                "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-100:])),
+                "train_tflops": mlp_train_flops(4096) / per_train / 1e12 if per_train > 0 else 0.0,
+                "first_max": float(d1.max()), "first_mean": float(d1.mean()),
+                "first_frac_gt_1e-3": float((d1 > 1e-3).double().mean()),
+                "refined_max": float(d2.max()), "refined_mean": float(d2.mean()),
+                "refined_frac_gt_1e-3": float((d2 > 1e-3).double().mean())}
+
     flops1 = solve_flops(s1.evals, s1.iterations, L)
-    achieved = flops1 / (s1.solve_ms * 1e-3) / 1e12
+    flops2 = solve_flops(s2.evals, s2.iterations, L)
     fbytes = filter_bytes(n, s2.filter_channels)
     fgbs = fbytes / (s2.filter_ms * 1e-3) / 1e9 if s2.filter_ms > 0 else 0.0
+    # the dominant kernel of the reported step: the longer of the two solves
+    k1, k2 = (warm["s1"], warm["s2"]) if warm else (s1, s2)
+    if k2.solve_ms >= k1.solve_ms:
+        dom, dflops, dms = "refinement solve (FP64)", solve_flops(k2.evals, k2.iterations, L), k2.solve_ms
+    else:
+        dom, dflops, dms = "first-pass solve (FP64)", solve_flops(k1.evals, k1.iterations, L), k1.solve_ms
+    achieved = dflops / (dms * 1e-3) / 1e12
+    ms = warm["ms_per_step"] if warm else ms_plain
+    value = n / (ms * 1e-3) / 1e6
 
     rec = {
         "metric": "Mpixel/s unmix+refine, plain vs MLP warm-start at equal error; filter HBM GB/s",
@@ -138,23 +220,62 @@ def main():
         "data": "synthetic (seeded generator, unblending_amd/synth.py)",
         "config": {"workload": wl.name, "width": w, "height": h, "layers": L,
                    "modes": [synth.MODE_NAMES[l.mode] for l in wl.layers], "filter_radius": wl.radius,
-                   "filter_eps": wl.eps, "warm_start": "none", "solver": "plain"},
-        "roofline": {"bound": "valu", "kernel": "first-pass solve (FP64)", "achieved": round(achieved, 3),
+                   "filter_eps": wl.eps, "warm_start": "mlp" if warm else "none",
+                   "solver": "persistent" if args.solver == 0 else "simple"},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "launch_ms": round(s1.solve_ms, 3),
-                     "flops_per_launch": flops1},
+                     "traffic": None, "launch_ms": round(dms, 3), "flops_per_launch": dflops},
         "filter": {"achieved": round(fgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(fgbs / HBM_PEAK_GBS, 4), "ms": round(s2.filter_ms, 3),
                    "coeff_ms": round(s2.filter_coeff_ms, 3), "out_ms": round(s2.filter_out_ms, 3),
                    "bytes": fbytes, "channels": s2.filter_channels},
-        "solve": {"unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
+        "plain": {"value": round(n / (ms_plain * 1e-3) / 1e6, 4), "ms_per_step": round(ms_plain, 3),
+                  "unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
                   "unmix_evals_per_px": s1.evals / n, "unmix_iters_per_px": s1.iterations / n,
                   "unmix_outer_per_px": s1.outer / n, "refine_evals_per_px": s2.evals / n,
-                  "refine_tflops": round(solve_flops(s2.evals, s2.iterations, L) / (s2.solve_ms * 1e-3) / 1e12, 3)},
+                  "unmix_tflops": round(flops1 / (s1.solve_ms * 1e-3) / 1e12, 3),
+                  "refine_tflops": round(flops2 / (s2.solve_ms * 1e-3) / 1e12, 3)},
     }
+    if warm:
+        w1, w2 = warm["s1"], warm["s2"]
+        rec["warm_start"] = {
+            "speedup": round(ms_plain / warm["ms_per_step"], 4),
+            "unmix_ms": round(w1.solve_ms, 3), "predict_ms": round(w1.predict_ms, 3),
+            "refine_ms": round(w2.solve_ms, 3),
+            "unmix_evals_per_px": w1.evals / n, "unmix_iters_per_px": w1.iterations / n,
+            "evals_saved_frac": round(1.0 - w1.evals / s1.evals, 4),
+            "iterations_saved_frac": round(1.0 - w1.iterations / s1.iterations, 4),
+            "fallback_pixels": int(w1.fallback_pixels),
+            "refine_evals_per_px": w2.evals / n,
+            "vs_plain_first_pass": {"max": warm["first_max"], "mean": warm["first_mean"],
+                                    "frac_gt_1e-3": warm["first_frac_gt_1e-3"]},
+            "vs_plain_refined": {"max": warm["refined_max"], "mean": warm["refined_mean"],
+                                 "frac_gt_1e-3": warm["refined_frac_gt_1e-3"]},
+            "training": {"images": args.train_images, "size": args.train_size, "steps": args.train_steps,
+                         "batch": 4096, "lr": args.train_lr, "label_s": round(warm["label_s"], 3),
+                         "train_s": round(warm["train_s"], 3), "loss_first": warm["loss_first"],
+                         "loss_last": warm["loss_last"], "mfma_tflops": round(warm["train_tflops"], 3),
+                         "mfma_frac_bf16": round(warm["train_tflops"] / BF16_PEAK_TFLOPS, 5)},
+        }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        rec["cpu_baseline"] = cpu_baseline(wl, args.cpu_rows, threads)
+        cpu, ofirst, osecond = cpu_baseline(wl, args.cpu_rows, threads)
+        rows = args.cpu_rows
+        keep = max(0, rows - 2 * wl.radius - 2)  # refinement rows unaffected by the band's lower edge
+        g1 = first_plain[:, :rows].cpu().numpy()
+        g2 = out[:, :keep].cpu().numpy()
+        cpu["plain_vs_oracle"] = {"first_max": float(np.abs(g1 - ofirst).max()),
+                                  "refined_max": float(np.abs(g2 - osecond[:, :keep]).max()) if keep else None,
+                                  "refined_rows": keep}
+        if warm:
+            gw1 = first[:, :rows].cpu().numpy()
+            gw2 = out_warm[:, :keep].cpu().numpy()
+            cpu["warm_vs_oracle"] = {"first_max": float(np.abs(gw1 - ofirst).max()),
+                                     "first_frac_gt_1e-3": float(np.mean(np.abs(gw1 - ofirst) > 1e-3)),
+                                     "refined_max": float(np.abs(gw2 - osecond[:, :keep]).max()) if keep else None,
+                                     "refined_frac_gt_1e-3":
+                                         float(np.mean(np.abs(gw2 - osecond[:, :keep]) > 1e-3)) if keep else None}
+        rec["cpu_baseline"] = cpu
     print(json.dumps(rec), flush=True)
 
 
