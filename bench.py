@@ -91,12 +91,23 @@ def main():
     ap.add_argument("--train-size", type=int, default=512)
     ap.add_argument("--train-images", type=int, default=4)
     ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (the reference point "
+                         "for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 8}), flush=True)
+        return
     import unblending_amd as U
     from unblending_amd import device, synth
 

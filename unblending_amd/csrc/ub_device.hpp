@@ -226,23 +226,28 @@ struct Shape {
     static constexpr int NC = REFINE ? 3 + L : 4;
 };
 
-// Augmented-Lagrangian objective, its gradient and the constraint vector at x.
+// Augmented-Lagrangian objective and its gradient at x.
 // x = [alpha_0..alpha_{L-1}, c_0 rgb .. c_{L-1} rgb].  The forward sweep
 // composites bottom to top and accumulates the energy and its colour
 // gradient; the reverse sweep applies the transposed 4x4 blocks of
 // equations.cpp:222-282 (one reciprocal of A per layer) to rho C - lambda.
-template <int L, bool REFINE>
-__device__ __forceinline__ double objective(const double* __restrict__ x, const DevLayer* lay,
-                                            const double tgt[3], const double* ta,
-                                            const double* lam, const double rho,
-                                            double* __restrict__ grad, double* __restrict__ C)
+// tgt, ta and lam are indexable (plain arrays, or per-lane LDS vectors); in
+// refinement the alpha slots of grad hold each layer's distance between the
+// two sweeps, which keeps no separate array live.  Cc receives the
+// constraints that depend on the composite: the colour residual (3) and, in
+// the first pass, the alpha residual (the refinement's alpha constraints
+// alpha_i - t_i need no composite).
+template <int L, bool REFINE, class TG, class TA, class TL>
+__device__ __forceinline__ double objective(const double* __restrict__ x, const DevLayer* lay, const TG& tgt,
+                                            const TA& ta, const TL& lam, const double rho,
+                                            double* __restrict__ grad, double* __restrict__ Cc)
 {
     constexpr int NC = Shape<L, REFINE>::NC;
     const double* alphas = x;
     const double* colors = x + L;
 
     double pre[L][4];
-    double dist[L];
+    double C[NC];
     double energy = 0.0;
 #pragma unroll
     for (int k = 0; k < L; ++k) {
@@ -268,8 +273,9 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
             }
             pre[k][3] = a;
         }
-        dist[k] = distance(ck[0], ck[1], ck[2], l);
-        energy += alphas[k] * dist[k];
+        const double dist = distance(ck[0], ck[1], ck[2], l);
+        if constexpr (REFINE) grad[k] = dist;
+        energy += alphas[k] * dist;
         double g[3];
         distance_gradient(ck[0], ck[1], ck[2], l, g);
 #pragma unroll
@@ -288,6 +294,8 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         for (int i = 0; i < L; ++i) sum += alphas[i];
         energy += 0.01 * sum;
     }
+#pragma unroll
+    for (int j = 0; j < (REFINE ? 3 : 4); ++j) Cc[j] = C[j];
     double lagrange = 0.0, squared = 0.0;
 #pragma unroll
     for (int j = 0; j < NC; ++j) lagrange += lam[j] * C[j];
@@ -329,7 +337,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         }
         double ga = adj_as + pA_s * aA;
         if constexpr (REFINE) ga = ga + w[3 + k];
-        grad[k] = (REFINE ? dist[k] : 0.01) + ga;
+        grad[k] = (REFINE ? grad[k] : 0.01) + ga;
         aA = adj_ad + pA_d * aA;
         aB[0] = nB[0];
         aB[1] = nB[1];
@@ -337,7 +345,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     }
     double ga0 = aA;
     if constexpr (REFINE) ga0 = ga0 + w[3];
-    grad[0] = (REFINE ? dist[0] : 0.01) + ga0;
+    grad[0] = (REFINE ? grad[0] : 0.01) + ga0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) grad[L + c] = grad[L + c] + aB[c];
     return f;

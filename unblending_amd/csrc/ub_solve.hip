@@ -60,9 +60,10 @@ struct Problem {
     double ta[L];
     double lam[Shape<L, REFINE>::NC];
     double rho;
-    __device__ __forceinline__ double eval(const double* x, double* g, double* C) const
+    __device__ __forceinline__ double eval(const double* x, double* g) const
     {
-        return objective<L, REFINE>(x, lay, tgt, ta, lam, rho, g, C);
+        double cc[4];
+        return objective<L, REFINE>(x, lay, tgt, ta, lam, rho, g, cc);
     }
 };
 
@@ -72,8 +73,7 @@ __device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L
                          const SolverParams& sp, int& nev_out, int& iters_out)
 {
     constexpr int N = 4 * L;
-    constexpr int NC = Shape<L, REFINE>::NC;
-    double g[N], d[N], q[N], xt[N], gt[N], C[NC];
+    double g[N], d[N], q[N], xt[N], gt[N];
     double S[M][N], Yh[M][N], rh[M], a[M];
     bool freev[N];
     int count = 0;
@@ -85,7 +85,7 @@ __device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L
         bd.get(i, lo, hi);
         x[i] = clampd(x[i], lo, hi);
     }
-    double f = pb.eval(x, g, C);
+    double f = pb.eval(x, g);
     int nev = 1, iters = 0;
 
     for (;;) {
@@ -161,7 +161,7 @@ __device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L
                 bd.get(i, lo, hi);
                 xt[i] = clampd(x[i] + t * d[i], lo, hi);
             }
-            ft = pb.eval(xt, gt, C);
+            ft = pb.eval(xt, gt);
             ++nev;
             double dphi = 0.0;
 #pragma unroll
@@ -233,9 +233,9 @@ __device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L
 }
 
 // Constraint vector alone (equations.cpp:113-155), for the outer loop.
-template <int L, bool REFINE>
-__device__ __forceinline__ void constraints(const double* x, const DevLayer* lay, const double tgt[3],
-                                            const double* ta, double* C)
+template <int L, bool REFINE, class TG, class TA>
+__device__ __forceinline__ void constraints(const double* x, const DevLayer* lay, const TG& tgt, const TA& ta,
+                                            double* C)
 {
     const double* alphas = x;
     const double* colors = x + L;
@@ -449,27 +449,75 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // hold the trial point and trial gradient between a search direction and its
 // acceptance (the pair is no longer needed once the direction is computed).
 // LDS holds, per lane, the gradient, the search direction, the multipliers
-// and the constraint vector.  Only the outer iterate, read and written once
-// per outer iteration, has a per-lane global slot; state touched every
-// evaluation stays on chip, where it costs no memory traffic.
+// the constraints that depend on the composite at the current point (kept
+// from that point's evaluation) and, in refinement where the layer count
+// leaves room, the pixel's target colour and target alphas, so that an
+// evaluation's live set fits the 256 architectural registers of a lane.
+// Only the outer iterate, read and written once per outer iteration, has a
+// per-lane global slot.
 // ---------------------------------------------------------------------------
 enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kIdle = 3 };
+
+// A per-lane vector in LDS, laid out [element][64 lanes].
+struct LaneVec {
+    double* p;
+    __device__ __forceinline__ double& operator[](int i) const { return p[i * 64]; }
+};
+
+// The pixel's constant inputs read by every evaluation, the target colour and
+// the target alphas (refinement), in LDS or in registers.
+template <int L, bool IN_LDS>
+struct PixelConst;
+
+template <int L>
+struct PixelConst<L, true> {
+    static constexpr int kCount = 3 + L;
+    LaneVec tgt, ta;
+    __device__ __forceinline__ PixelConst(double* base) : tgt{base}, ta{base + 3 * 64} {}
+};
+
+template <int L>
+struct PixelConst<L, false> {
+    static constexpr int kCount = 1;
+    double tgt[3], ta[L];
+    __device__ __forceinline__ PixelConst(double*) {}
+};
+
+template <int L, bool REFINE>
+struct PersistentShape {
+    static constexpr int N = 4 * L;
+    static constexpr int NC = Shape<L, REFINE>::NC;
+    static constexpr int NCC = REFINE ? 3 : 4;  // constraints that need the composite
+    // doubles of LDS a lane may use with four workgroups of 64 lanes on a CU
+    static constexpr int kLaneBudget = 160 * 1024 / (4 * 64 * 8);
+    // the refinement's larger live set leaves no registers for the pixel's
+    // targets; the first pass keeps them in registers
+    static constexpr bool kPixelInLds = REFINE && 2 * N + NC + NCC + (3 + L) <= kLaneBudget;
+};
 
 template <int L, bool REFINE, int M>
 __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
 {
     static_assert(M == 1, "the persistent kernel keeps one (s, y) pair");
-    constexpr int N = 4 * L;
-    constexpr int NC = Shape<L, REFINE>::NC;
+    using PS = PersistentShape<L, REFINE>;
+    constexpr int N = PS::N;
+    constexpr int NC = PS::NC;
+    using PC = PixelConst<L, PS::kPixelInLds>;
     __shared__ double s_g[N][64];    // gradient at the current point
     __shared__ double s_d[N][64];    // search direction
     __shared__ double s_lam[NC][64]; // multipliers
-    __shared__ double s_c[NC][64];   // constraint vector at the current point
+    __shared__ double s_c[PS::NCC][64];  // composite constraints at the current point
+    __shared__ double s_px[PC::kCount][64];
     const int lane = threadIdx.x;
     // the outer iterate, touched once per outer iteration: a per-lane global
-    // slot, [variable][lane]
-    const long long stride = (long long)gridDim.x * 64;
-    double* xo = args.lane_state + (long long)blockIdx.x * 64 + lane;
+    // slot, [workgroup][variable][lane]; its address is formed inside the
+    // loop (from the opaque zero below) so that the compiler does not keep N
+    // precomputed 64-bit addresses live across the loop
+    double* const xo_base = args.lane_state + (long long)blockIdx.x * (N * 64) + lane;
+    const LaneVec lam{&s_lam[0][lane]};
+    const LaneVec cc{&s_c[0][lane]};
+    PC pc(&s_px[0][lane]);
+    double bg[3] = {0.0, 0.0, 0.0};  // background colour (smoothed refinement)
 
     const SolverParams sp = args.sp;
     const bool opaque = args.opaque != 0;
@@ -480,7 +528,6 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
     // point and trial gradient.
     double xc[N], S[N], Y[N];
     double rh = 0.0;
-    double tgt[3], ta[L], bg[3];
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
     int phase = kNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
     long long pix = 0;
@@ -493,15 +540,15 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         Y[i] = 0.0;
     }
 #pragma unroll
-    for (int l = 0; l < L; ++l) ta[l] = 0.0;
-    tgt[0] = tgt[1] = tgt[2] = 0.0;
-    bg[0] = bg[1] = bg[2] = 0.0;
+    for (int l = 0; l < L; ++l) pc.ta[l] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pc.tgt[c] = 0.0;
 
     auto bounds = [&](int i, double& lo, double& hi) {
         lo = 0.0;
         hi = 1.0;
         if (i < L) {
-            if (afixed) { lo = ta[i]; hi = ta[i]; }
+            if (afixed) { lo = pc.ta[i]; hi = pc.ta[i]; }
             if (i == 0 && opaque) {
                 lo = 1.0;
                 if (afixed) hi = 1.0;
@@ -523,6 +570,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         int zero;
         asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
         const DevLayer* lay = args.layers + zero;
+        double* const xo = xo_base + zero;
 
         // ---- next pixels for the lanes that finished theirs ----
         const unsigned long long needy = __ballot(phase == kNeedPixel);
@@ -537,19 +585,23 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                     phase = kIdle;
                 } else {
                     pix = args.list ? args.list[item] : item;
-                    double x0[N], C0[NC];
-                    load_pixel<L, REFINE>(args, lay, pix, tgt, ta, bg, x0);
+                    double x0[N], C0[NC], tg0[3], ta0[L];
+                    load_pixel<L, REFINE>(args, lay, pix, tg0, ta0, bg, x0);
 #pragma unroll
-                    for (int j = 0; j < NC; ++j) s_lam[j][lane] = 0.0;
+                    for (int c = 0; c < 3; ++c) pc.tgt[c] = tg0[c];
+#pragma unroll
+                    for (int l = 0; l < L; ++l) pc.ta[l] = ta0[l];
+#pragma unroll
+                    for (int j = 0; j < NC; ++j) lam[j] = 0.0;
                     rho = sp.rho0;
                     ocount = 0;
-                    constraints<L, REFINE>(x0, lay, tgt, ta, C0);
+                    constraints<L, REFINE>(x0, lay, tg0, ta0, C0);
                     cn_outer = norm<NC>(C0);
                     count = 0;
                     gamma = 1.0;
 #pragma unroll
                     for (int i = 0; i < N; ++i) {
-                        xo[i * stride] = x0[i];
+                        xo[i * 64] = x0[i];
                         double lo, hi;
                         bounds(i, lo, hi);
                         S[i] = clampd(x0[i], lo, hi);
@@ -561,10 +613,8 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         if (__ballot(phase != kIdle) == 0) break;
 
         // ---- one evaluation per lane at the trial point S ----
-        double lam[NC], Ct[NC];
-#pragma unroll
-        for (int j = 0; j < NC; ++j) lam[j] = s_lam[j][lane];
-        const double ft = objective<L, REFINE>(S, lay, tgt, ta, lam, rho, Y, Ct);
+        double Ct[PS::NCC];
+        const double ft = objective<L, REFINE>(S, lay, pc.tgt, pc.ta, lam, rho, Y, Ct);
 
         bool direction = false, inner_end = false;
         if (phase == kStart) {
@@ -576,7 +626,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 s_g[i][lane] = Y[i];
             }
 #pragma unroll
-            for (int j = 0; j < NC; ++j) s_c[j][lane] = Ct[j];
+            for (int j = 0; j < PS::NCC; ++j) cc[j] = Ct[j];
             direction = true;
         } else if (phase == kTrial) {
             ++nev;
@@ -603,7 +653,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 const double fold = f;
                 f = ft;
 #pragma unroll
-                for (int j = 0; j < NC; ++j) s_c[j][lane] = Ct[j];
+                for (int j = 0; j < PS::NCC; ++j) cc[j] = Ct[j];
                 if (sy > 2.220446049250313e-16 * yy) {
                     rh = 1.0 / sy;
                     gamma = sy / yy;
@@ -714,20 +764,25 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         if (inner_end) {
             c_evals += (unsigned long long)nev;
             c_outer += 1;
+            // the constraint vector at the current point: the composite part
+            // kept from its evaluation, the refinement's alpha part alpha_i - t_i
             double Ccur[NC];
 #pragma unroll
-            for (int j = 0; j < NC; ++j) {
-                Ccur[j] = s_c[j][lane];
-                s_lam[j][lane] = s_lam[j][lane] - rho * Ccur[j];
+            for (int j = 0; j < PS::NCC; ++j) Ccur[j] = cc[j];
+            if constexpr (REFINE) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) Ccur[3 + i] = xc[i] - pc.ta[i];
             }
+#pragma unroll
+            for (int j = 0; j < NC; ++j) lam[j] = lam[j] - rho * Ccur[j];
             const double ngn = norm<NC>(Ccur);
             if (ngn > sp.gamma * cn_outer) rho = rho * sp.beta;
             double dx2 = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const double dxv = xc[i] - xo[i * stride];
+                const double dxv = xc[i] - xo[i * 64];
                 dx2 += dxv * dxv;
-                xo[i * stride] = xc[i];
+                xo[i * 64] = xc[i];
             }
             const bool unchanged = __builtin_sqrt(dx2) < sp.eps;
             const bool satisfied = ngn < sp.eps;
