@@ -44,10 +44,14 @@ typedef struct unblending_hip_params {
     int32_t warm_start;       /* first pass: 1 = start from the model's prediction (mlp), re-solve
                                  from the default initialisation where that misses the tolerances */
     const void* mlp;          /* device buffer of a trained model (unblending_hip_mlp_*) */
+    int32_t chunk_size;       /* pixels a wave takes from the solve queue at once; 0: default */
+    int32_t filter_block;     /* guided filter workgroup width (256, 512, 1024); 0: chosen by radius */
+    int32_t filter_rows;      /* guided filter rows per workgroup; 0: chosen by image size */
+    int32_t reserved2;
 } unblending_hip_params;
 
 #define UNBLENDING_HIP_PARAMS_INIT \
-    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL }
+    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL, 0, 0, 0, 0 }
 
 typedef struct unblending_hip_stats {
     uint64_t evals;        /* objective+gradient evaluations, summed over pixels */
@@ -135,6 +139,39 @@ UNBLENDING_API size_t unblending_hip_mlp_predict_workspace_size(void);
 UNBLENDING_API int32_t unblending_hip_mlp_predict(
     const void* model, const double* rgb, int32_t width, int32_t height, int32_t layers,
     double* output, void* workspace, size_t workspace_size, void* stream);
+
+/* ---- batches of images (no counterpart in the reference, which solves one
+ * image a call, unblending/sdk/unblending_c.cpp:50-110) ----
+ * Device-resident images of any sizes, each with its own layer set (1 to 8
+ * layers).  All images of one layer count are solved by one persistent
+ * launch whose waves take pixel chunks of those images from a device queue;
+ * the per-pixel results are those of unblending_hip_unmix_device /
+ * unblending_hip_refine_device on each image alone.  The options apply to
+ * every image.  Enqueued on `stream`; no allocation, no synchronisation
+ * unless stats are asked for, so a call can be captured in a HIP graph. */
+typedef struct unblending_hip_image {
+    const double* rgb;               /* device [height][width][3] */
+    int32_t width, height;
+    const unblending_layer* layers;  /* host */
+    size_t layer_count;
+    const double* first_pass;        /* refinement: device packed first pass */
+    double* alpha_planes;            /* first pass: optional output alpha planes [L][h][w];
+                                        refinement: optional alpha planes of the first pass */
+    double* output;                  /* device packed [layer][height][width][RGBA] */
+    int32_t filter_radius;           /* refinement: < 0: 60 * min(width, height) / 1000 */
+    int32_t reserved;
+} unblending_hip_image;
+
+UNBLENDING_API size_t unblending_hip_batch_workspace_size(const unblending_hip_image* images, size_t count,
+                                                         int32_t refine, int32_t smooth_background);
+UNBLENDING_API int32_t unblending_hip_unmix_batch_device(
+    const unblending_hip_image* images, size_t count, const unblending_options* options,
+    const unblending_hip_params* params, void* workspace, size_t workspace_size, void* stream,
+    unblending_hip_stats* stats);
+UNBLENDING_API int32_t unblending_hip_refine_batch_device(
+    const unblending_hip_image* images, size_t count, const unblending_options* options,
+    const unblending_hip_params* params, void* workspace, size_t workspace_size, void* stream,
+    unblending_hip_stats* stats);
 
 /* Planar seam, host pointers (see above).  Returns an unblending status. */
 UNBLENDING_API int32_t uhip_solve_planes(

@@ -221,3 +221,56 @@ def test_warm_start_model_interface_checks():
     assert lib.unblending_hip_mlp_train(buf, buf, 1, 4, 4, buf, 3, 1, 0, 128, 1e-3, 0, None, 1, buf, 16,
                                         None, None, None) == U.BUFFER_TOO_SMALL
     assert lib.unblending_hip_mlp_set_params(None, None, None) == U.INVALID_ARGUMENT
+
+
+def _batch_images(sizes, n_layers):
+    arr = (U.HipImage * len(sizes))()
+    keep = []
+    for i, ((w, h), L) in enumerate(zip(sizes, n_layers)):
+        lay = layers(L)
+        keep.append(lay)
+        arr[i].rgb = 16
+        arr[i].width, arr[i].height = w, h
+        arr[i].layers = ctypes.cast(lay, ctypes.c_void_p)
+        arr[i].layer_count = L
+        arr[i].first_pass = 16
+        arr[i].output = 16
+        arr[i].filter_radius = -1
+    return arr, keep
+
+
+def test_batch_entry_points_validate():
+    lib = U.lib()
+    imgs, keep = _batch_images([(4, 3), (5, 7), (2, 2)], [3, 5, 3])
+    ws1 = lib.unblending_hip_batch_workspace_size(imgs, 3, 0, 0)
+    ws2 = lib.unblending_hip_batch_workspace_size(imgs, 3, 1, 0)
+    ws3 = lib.unblending_hip_batch_workspace_size(imgs, 3, 1, 1)
+    assert 0 < ws1 < ws2 < ws3
+    assert lib.unblending_hip_batch_workspace_size(None, 3, 0, 0) == 0
+    assert lib.unblending_hip_batch_workspace_size(imgs, 0, 0, 0) == 0
+    opt = U.options(U.BACKEND_HIP)
+    buf = ctypes.c_void_p(16)  # never dereferenced: every call below fails its argument checks first
+    unmix, refine = lib.unblending_hip_unmix_batch_device, lib.unblending_hip_refine_batch_device
+    assert unmix(None, 3, ctypes.byref(opt), None, buf, ws1, None, None) == U.INVALID_ARGUMENT
+    assert unmix(imgs, 0, ctypes.byref(opt), None, buf, ws1, None, None) == U.INVALID_ARGUMENT
+    assert unmix(imgs, 3, ctypes.byref(opt), None, buf, ws1 - 1, None, None) == U.BUFFER_TOO_SMALL
+    warm = U.hip_params()
+    warm.warm_start = 1
+    assert unmix(imgs, 3, ctypes.byref(opt), ctypes.byref(warm), buf, ws1, None, None) == U.INVALID_ARGUMENT
+    simple = U.hip_params(solver=1)
+    assert unmix(imgs, 3, ctypes.byref(opt), ctypes.byref(simple), buf, ws1, None, None) == U.INVALID_ARGUMENT
+    # image checks in image order: a bad layer set, a missing first pass, too many layers
+    imgs[1].layer_count = 0
+    assert unmix(imgs, 3, ctypes.byref(opt), None, buf, ws1, None, None) == U.INVALID_ARGUMENT
+    imgs[1].layer_count = 5
+    keep[1][2].precision[1] = 7.0  # asymmetric precision
+    assert unmix(imgs, 3, ctypes.byref(opt), None, buf, ws1, None, None) == U.INVALID_ARGUMENT
+    keep[1][2].precision[1] = 0.0
+    imgs[2].first_pass = None
+    assert refine(imgs, 3, ctypes.byref(opt), None, buf, ws3, None, None) == U.INVALID_ARGUMENT
+    imgs[2].first_pass = 16
+    smooth_transparent = U.options(U.BACKEND_HIP, opaque=False, smooth=True)
+    assert refine(imgs, 3, ctypes.byref(smooth_transparent), None, buf, ws3, None, None) == U.INVALID_ARGUMENT
+    nine, keep9 = _batch_images([(4, 3)], [9])
+    assert unmix(nine, 1, ctypes.byref(opt), None, buf, 1 << 40, None, None) == U.DEVICE_ERROR
+    assert "1 to 8 layers" in U.last_error()

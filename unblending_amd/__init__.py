@@ -150,7 +150,16 @@ class HipParams(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("filter_radius", ctypes.c_int32),
                 ("filter_epsilon", ctypes.c_double), ("alpha_fixed", ctypes.c_int32),
                 ("solver", ctypes.c_int32), ("timing", ctypes.c_int32), ("warm_start", ctypes.c_int32),
-                ("mlp", ctypes.c_void_p)]
+                ("mlp", ctypes.c_void_p), ("chunk_size", ctypes.c_int32), ("filter_block", ctypes.c_int32),
+                ("filter_rows", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
+
+
+class HipImage(ctypes.Structure):
+    """unblending_hip_image: one image of a batch (device pointers)."""
+    _fields_ = [("rgb", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("layers", ctypes.c_void_p), ("layer_count", ctypes.c_size_t), ("first_pass", ctypes.c_void_p),
+                ("alpha_planes", ctypes.c_void_p), ("output", ctypes.c_void_p), ("filter_radius", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class HipStats(ctypes.Structure):
@@ -227,6 +236,13 @@ _lib.unblending_hip_mlp_predict_workspace_size.argtypes = []
 _lib.unblending_hip_mlp_predict.restype = ctypes.c_int32
 _lib.unblending_hip_mlp_predict.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
                                             ctypes.c_size_t, _vp]
+_lib.unblending_hip_batch_workspace_size.restype = ctypes.c_size_t
+_lib.unblending_hip_batch_workspace_size.argtypes = [ctypes.POINTER(HipImage), ctypes.c_size_t, ctypes.c_int32,
+                                                     ctypes.c_int32]
+for _f in (_lib.unblending_hip_unmix_batch_device, _lib.unblending_hip_refine_batch_device):
+    _f.restype = ctypes.c_int32
+    _f.argtypes = [ctypes.POINTER(HipImage), ctypes.c_size_t, ctypes.POINTER(Options), ctypes.POINTER(HipParams),
+                   _vp, ctypes.c_size_t, _vp, ctypes.POINTER(HipStats)]
 _lib.unblending_hip_last_error.restype = ctypes.c_char_p
 
 
@@ -268,8 +284,11 @@ def options(backend=BACKEND_AUTO, opaque=True, smooth=False, target_concurrency=
     return o
 
 
-def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, warm_start=None) -> HipParams:
-    """warm_start: a trained ``mlp.WarmStartModel`` (first pass only), or None."""
+def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, warm_start=None, chunk_size=0,
+               filter_block=0, filter_rows=0) -> HipParams:
+    """warm_start: a trained ``mlp.WarmStartModel`` (first pass only), or None.
+    chunk_size / filter_block / filter_rows: launch shapes (0: defaults; see
+    ``unblending_amd.autotune``)."""
     p = HipParams()
     p.struct_size = ctypes.sizeof(HipParams)
     p.filter_radius = int(radius)
@@ -279,6 +298,9 @@ def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, wa
     p.timing = int(bool(timing))
     p.warm_start = int(warm_start is not None)
     p.mlp = warm_start.data_ptr() if warm_start is not None else None
+    p.chunk_size = int(chunk_size)
+    p.filter_block = int(filter_block)
+    p.filter_rows = int(filter_rows)
     return p
 
 

@@ -264,6 +264,66 @@ int32_t device_call(const double* rgb, int32_t width, int32_t height, const unbl
     }
 }
 
+// Batches: the checks of device_call for every image, in image order.
+int32_t batch_call(const unblending_hip_image* images, size_t count, const unblending_options* options,
+                   const unblending_hip_params* params, bool refine, void* workspace, size_t workspace_size,
+                   void* stream, unblending_hip_stats* stats)
+{
+    g_last_error.clear();
+    try {
+        constexpr auto max_int = std::numeric_limits<int>::max();
+        if (!images || count == 0) return UNBLENDING_INVALID_ARGUMENT;
+        const unblending_options defaults = UNBLENDING_OPTIONS_INIT;
+        if (!options) options = &defaults;
+        if (options->struct_size < sizeof(unblending_options) ||
+            (options->opaque_background != 0 && options->opaque_background != 1) ||
+            (options->smooth_background != 0 && options->smooth_background != 1) ||
+            options->target_concurrency < 0)
+            return UNBLENDING_INVALID_ARGUMENT;
+        unblending_hip_params p = UNBLENDING_HIP_PARAMS_INIT;
+        if (params) {
+            if (params->struct_size < sizeof(unblending_hip_params)) return UNBLENDING_INVALID_ARGUMENT;
+            p = *params;
+        }
+        if (p.warm_start != 0 || p.solver != 0 || p.chunk_size < 0) return UNBLENDING_INVALID_ARGUMENT;
+        const Backend backend = select_backend(options->backend);
+        if (backend == Backend::cpu) return UNBLENDING_INVALID_ARGUMENT;
+        if (backend == Backend::cuda) return UNBLENDING_DEVICE_ERROR;
+        const bool opaque = options->opaque_background != 0;
+        const bool smooth = refine && options->smooth_background != 0;
+        if (smooth && !opaque) return UNBLENDING_INVALID_ARGUMENT;
+        std::vector<LayerSet> sets;
+        sets.reserve(count);
+        for (size_t i = 0; i < count; ++i) {
+            const unblending_hip_image& im = images[i];
+            if (!im.rgb || !im.layers || !im.layer_count || im.layer_count > size_t(max_int) / 4 || im.width <= 0 ||
+                im.height <= 0 || im.width > max_int / im.height || !im.output || (refine && !im.first_pass))
+                return UNBLENDING_INVALID_ARGUMENT;
+            sets.push_back(make_layer_set(im.layers, im.layer_count));
+            if (refine && !sets.back().all_plus && !sets.back().all_so) return UNBLENDING_INVALID_ARGUMENT;
+            if (im.layer_count > size_t(kMaxLayers))
+                throw ExecutionError("Layer count is outside the device solver's domain (1 to 8 layers)");
+        }
+        if (!workspace || workspace_size < batch_workspace_bytes(images, count, refine, smooth))
+            return UNBLENDING_BUFFER_TOO_SMALL;
+        run_batch(images, sets, refine, opaque, smooth, p, workspace, static_cast<hipStream_t>(stream), stats);
+        return UNBLENDING_OK;
+    } catch (const std::bad_alloc&) {
+        return UNBLENDING_OUT_OF_MEMORY;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return UNBLENDING_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    } catch (...) {
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
 }  // namespace
 }  // namespace ubhip
 
@@ -337,6 +397,33 @@ UNBLENDING_API int32_t unblending_hip_refine_device(const double* rgb, int32_t w
     if (!first_pass) return UNBLENDING_INVALID_ARGUMENT;
     return device_call(rgb, width, height, layers, layer_count, first_pass, alpha_planes, options, params,
                        output, nullptr, workspace, workspace_size, stream, stats);
+}
+
+UNBLENDING_API size_t unblending_hip_batch_workspace_size(const unblending_hip_image* images, size_t count,
+                                                         int32_t refine, int32_t smooth_background)
+{
+    if (!images || count == 0) return 0;
+    for (size_t i = 0; i < count; ++i)
+        if (images[i].width <= 0 || images[i].height <= 0 || images[i].layer_count == 0) return 0;
+    return batch_workspace_bytes(images, count, refine != 0, refine != 0 && smooth_background != 0);
+}
+
+UNBLENDING_API int32_t unblending_hip_unmix_batch_device(const unblending_hip_image* images, size_t count,
+                                                         const unblending_options* options,
+                                                         const unblending_hip_params* params, void* workspace,
+                                                         size_t workspace_size, void* stream,
+                                                         unblending_hip_stats* stats)
+{
+    return batch_call(images, count, options, params, false, workspace, workspace_size, stream, stats);
+}
+
+UNBLENDING_API int32_t unblending_hip_refine_batch_device(const unblending_hip_image* images, size_t count,
+                                                          const unblending_options* options,
+                                                          const unblending_hip_params* params, void* workspace,
+                                                          size_t workspace_size, void* stream,
+                                                          unblending_hip_stats* stats)
+{
+    return batch_call(images, count, options, params, true, workspace, workspace_size, stream, stats);
 }
 
 UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_t height, int32_t channels)

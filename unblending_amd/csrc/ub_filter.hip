@@ -190,18 +190,30 @@ struct Geometry {
     int E, TW, RH, strips, chunks;
 };
 
-bool geometry(int w, int h, int r, Geometry& g)
+// Strip geometry: workgroup width E (its middle TW = E - 2r columns are
+// output columns) and rows per workgroup RH.  By default E is the smallest
+// width whose output part is at least half of it and RH aims at ~1024
+// workgroups, at least 4r + 2 rows (the vertical warm-up is 2r + 1 rows).
+// block / rows override them (the autotuner's knobs).
+bool geometry(int w, int h, int r, int block, int rows, Geometry& g)
 {
     if (r < 0 || r > kMaxFilterRadius) return false;
-    if (256 - 2 * r >= 128) g.E = 256;
+    if (block == 256 || block == 512 || block == 1024) g.E = block;
+    else if (256 - 2 * r >= 128) g.E = 256;
     else if (512 - 2 * r >= 256) g.E = 512;
     else g.E = 1024;
+    if (g.E - 2 * r < 1) g.E = 1024;
     g.TW = g.E - 2 * r;
     g.strips = (w + g.TW - 1) / g.TW;
-    int want = (1024 + g.strips - 1) / g.strips;
-    int rh = (h + want - 1) / want;
-    rh = rh < 4 * r + 2 ? 4 * r + 2 : rh;
-    rh = rh < 32 ? 32 : rh;
+    int rh;
+    if (rows > 0) {
+        rh = rows;
+    } else {
+        const int want = (1024 + g.strips - 1) / g.strips;
+        rh = (h + want - 1) / want;
+        rh = rh < 4 * r + 2 ? 4 * r + 2 : rh;
+        rh = rh < 32 ? 32 : rh;
+    }
     g.RH = rh;
     g.chunks = (h + rh - 1) / rh;
     return true;
@@ -235,7 +247,7 @@ hipError_t launch_pass(const FilterArgs& a, hipStream_t s)
 {
     Geometry geo;
     if (a.w <= 0 || a.h <= 0) return hipSuccess;
-    if (!geometry(a.w, a.h, a.r, geo)) return hipErrorInvalidValue;
+    if (!geometry(a.w, a.h, a.r, a.block, a.rows, geo)) return hipErrorInvalidValue;
     switch (geo.E) {
     case 256: return launch_e<PASS_A, 256>(a, geo, s);
     case 512: return launch_e<PASS_A, 512>(a, geo, s);

@@ -58,10 +58,18 @@ mlp::PredictBuffers mlp_predict_view(void* ws, size_t n = size_t(1) << 40);
 size_t filter_workspace_bytes(size_t n, int channels);
 void run_device(const DeviceJob& job);
 void run_filter(const double* rgb, int w, int h, const Channel* in, double* const* q, int channels, int radius,
-                double eps, double* ab, hipStream_t stream, unblending_hip_stats* stats, bool timing);
+                double eps, double* ab, hipStream_t stream, unblending_hip_stats* stats, bool timing, int block = 0,
+                int rows = 0);
+
+// Batches of device-resident images (unblending_hip_*_batch_device).
+size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth);
+void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& sets, bool refine, bool opaque,
+               bool smooth, const unblending_hip_params& params, void* workspace, hipStream_t stream,
+               unblending_hip_stats* stats);
 void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const double* first, bool opaque,
                 bool smooth, const unblending_hip_params& params, double* output, unblending_hip_stats* stats);
 
 hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream);
+hipError_t launch_store_desc(ImageDesc* dst, const ImageDesc& d, hipStream_t stream);
 
 }  // namespace ubhip

@@ -30,7 +30,9 @@ struct SolverParams {
 // launch, fallback launch); kFallback counts the pixels a warm-started solve
 // hands to the fallback launch.
 enum Counter : int {
-    kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6, kNumCounters = 8
+    kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6,
+    kGroupQueue = 8,  // batches: the queue of the launch of the g-th layer count is kGroupQueue + g
+    kNumCounters = 16
 };
 
 struct PixelArgs {
@@ -53,7 +55,49 @@ struct PixelArgs {
     int queue;             // counter slot of the next-item queue
 };
 
+// One-pixel-per-lane solve of one image (PixelArgs::solver = 1).
 hipError_t launch_solve(int L, bool refine, const PixelArgs& args, hipStream_t stream);
+
+// One image of a persistent solve launch: its layer set, buffers and the
+// range of the launch's pixel chunks it owns.  Lives in device memory and is
+// read-only during the launch.
+struct ImageDesc {
+    DevLayer layers[kMaxLayers];
+    const double* rgb;       // [n][3]
+    const double* first;     // refinement: first pass, packed [L][n][RGBA]
+    const double* filtered;  // refinement: guided-filter output planes [L (+3)][n]
+    double* out;             // packed [L][n][RGBA]
+    double* alpha_planes;    // optional copy of the solved alphas [L][n]
+    const double* warm;      // first pass: predicted first pass (warm start) or nullptr
+    long long* fallback;     // warm start: pixels to re-solve from the default initialisation
+    unsigned long long* fallback_count;
+    const long long* list;   // solve list[0 .. *list_count) instead of pixels 0 .. n-1
+    const unsigned long long* list_count;
+    long long n;             // pixels
+    long long chunk_base;    // the image's first chunk; its chunks are consecutive
+    int L, opaque, smooth, all_plus, all_so, pad;
+};
+
+// A persistent solve over a batch of images (one pass kind).  The images'
+// pixels are cut into chunks of chunk_size (chunk k of image i covers pixels
+// (k - chunk_base) * chunk_size ...); a wave takes chunks from the counter
+// counters[queue] and solves one image at a time.
+struct BatchArgs {
+    const ImageDesc* images;
+    int image_count;
+    int chunk_size;
+    long long chunk_count;
+    SolverParams sp;
+    double* lane_state;      // kLaneStateDoubles of per-lane scratch
+    unsigned long long* counters;
+    int queue;
+    int max_blocks;          // 0: as many as fit on the device
+};
+
+constexpr int kDefaultChunk = 64;  // pixels a wave takes from the queue at once
+
+// Every image of the batch must have L layers.
+hipError_t launch_batch_solve(int L, bool refine, const BatchArgs& args, hipStream_t stream);
 
 // One filtered channel: value(pixel) = base[pixel * stride].
 struct Channel {
@@ -71,6 +115,8 @@ struct FilterArgs {
     Channel in[kMaxGroup];// pass A inputs
     double* ab;           // pass A output / pass B input: [4 * group][n] (a_r, a_g, a_b, b)
     double* q[kMaxGroup]; // pass B outputs
+    int block;            // workgroup width (256, 512 or 1024; 0: chosen by radius)
+    int rows;             // rows per workgroup (0: chosen by image size)
 };
 
 // Pass A: box statistics and per-pixel coefficients; pass B: box of the

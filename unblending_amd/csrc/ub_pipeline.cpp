@@ -22,7 +22,11 @@ void check(hipError_t e, const char* what)
     if (e != hipSuccess) throw ExecutionError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-constexpr size_t kHeader = 256;  // counters
+// workspace header: counters at 0, then the image descriptors of the solve
+// launch and of the warm start's fallback launch
+constexpr size_t kDescOffset = 256;
+constexpr size_t kDescStride = (sizeof(ImageDesc) + 255) / 256 * 256;
+constexpr size_t kHeader = kDescOffset + 2 * kDescStride;
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
@@ -192,7 +196,8 @@ size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool w
 }
 
 void run_filter(const double* rgb, int w, int h, const Channel* in, double* const* q, int channels, int radius,
-                double eps, double* ab, hipStream_t stream, unblending_hip_stats* stats, bool timing)
+                double eps, double* ab, hipStream_t stream, unblending_hip_stats* stats, bool timing, int block,
+                int rows)
 {
     if (radius > kMaxFilterRadius)
         throw ExecutionError("Guided-filter radius is outside the device filter's domain");
@@ -212,6 +217,8 @@ void run_filter(const double* rgb, int w, int h, const Channel* in, double* cons
             fa.q[j] = q[g0 + j];
         }
         fa.ab = ab;
+        fa.block = block;
+        fa.rows = rows;
         ea.start(stream);
         check(launch_filter_coefficients(fa, stream), "guided filter (coefficients)");
         ea.stop(stream);
@@ -295,24 +302,62 @@ void run_device(const DeviceJob& job)
             in[l] = job.alpha_in ? Channel{job.alpha_in + l * n, 1} : Channel{job.first + 4 * l * n + 3, 4};
         for (int c = 0; c < (job.smooth ? 3 : 0); ++c) in[L + c] = Channel{job.first + c, 4};
         for (int c = 0; c < C; ++c) q[c] = qbase + size_t(c) * n;
-        run_filter(job.rgb, job.w, job.h, in.data(), q.data(), C, radius, eps, ab, job.stream, job.stats, timing);
+        run_filter(job.rgb, job.w, job.h, in.data(), q.data(), C, radius, eps, ab, job.stream, job.stats, timing,
+                   job.params.filter_block, job.params.filter_rows);
         args.first = job.first;
         args.filtered = qbase;
     }
 
+    if (warm && args.solver != 0) throw std::invalid_argument("The warm start runs on the persistent solve kernel");
+    ImageDesc desc{};
+    BatchArgs batch{};
+    ImageDesc* ddesc = reinterpret_cast<ImageDesc*>(ws + kDescOffset);
+    if (args.solver == 0) {
+        for (size_t l = 0; l < L; ++l) desc.layers[l] = args.layers[l];
+        desc.rgb = args.rgb;
+        desc.first = args.first;
+        desc.filtered = args.filtered;
+        desc.out = args.out;
+        desc.alpha_planes = args.alpha_planes;
+        desc.warm = args.warm;
+        desc.fallback = args.fallback;
+        desc.fallback_count = counters + kFallback;
+        desc.n = args.n;
+        desc.L = int(L);
+        desc.opaque = args.opaque;
+        desc.smooth = args.smooth;
+        desc.all_plus = args.all_plus;
+        desc.all_so = args.all_so;
+        check(launch_store_desc(ddesc, desc, job.stream), "image descriptor");
+        batch.images = ddesc;
+        batch.image_count = 1;
+        batch.chunk_size = job.params.chunk_size > 0 ? job.params.chunk_size : kDefaultChunk;
+        batch.chunk_count = (args.n + batch.chunk_size - 1) / batch.chunk_size;
+        batch.sp = args.sp;
+        batch.lane_state = args.lane_state;
+        batch.counters = counters;
+        batch.queue = kQueue;
+    }
+
     Events es(timing);
     es.start(job.stream);
-    if (warm && args.solver != 0) throw std::invalid_argument("The warm start runs on the persistent solve kernel");
-    check(launch_solve(int(L), refine, args, job.stream), "solve");
+    if (args.solver == 0) check(launch_batch_solve(int(L), refine, batch, job.stream), "solve");
+    else check(launch_solve(int(L), refine, args, job.stream), "solve");
     if (warm) {
         // re-solve, from the default initialisation, the pixels whose
         // warm-started solve missed the tolerances
-        PixelArgs fb = args;
+        ImageDesc fb = desc;
         fb.warm = nullptr;
-        fb.list = args.fallback;
         fb.fallback = nullptr;
-        fb.queue = kQueue2;
-        check(launch_solve(int(L), false, fb, job.stream), "fallback solve");
+        fb.fallback_count = nullptr;
+        fb.list = args.fallback;
+        fb.list_count = counters + kFallback;
+        ImageDesc* dfb = reinterpret_cast<ImageDesc*>(ws + kDescOffset + kDescStride);
+        check(launch_store_desc(dfb, fb, job.stream), "image descriptor");
+        BatchArgs fbatch = batch;
+        fbatch.images = dfb;
+        fbatch.queue = kQueue2;
+        check(launch_batch_solve(int(L), false, fbatch, job.stream), "fallback solve");
     }
     es.stop(job.stream);
     if (job.stats) {
@@ -327,6 +372,171 @@ void run_device(const DeviceJob& job)
         job.stats->fallback_pixels = host_counters[kFallback];
         job.stats->solve_ms = es.ms();
         job.stats->predict_ms = predict_ms;
+    }
+}
+
+namespace {
+
+// Workspace of a batch call: counters, one descriptor per image, the solve's
+// lane state, the filter's coefficient planes (shared by the images, filtered
+// one after another) and every image's filtered planes (image i's start
+// q_offset[i] doubles into the planes area).
+struct BatchLayout {
+    size_t desc, lane_state, ab, planes, total;
+    std::vector<size_t> q_offset;
+};
+
+BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool refine, bool smooth)
+{
+    BatchLayout b;
+    b.desc = kDescOffset;
+    b.lane_state = align256(b.desc + count * kDescStride);
+    b.ab = b.lane_state + align256(size_t(kLaneStateDoubles) * sizeof(double));
+    size_t ab_bytes = 0, planes = 0;
+    b.q_offset.assign(count, 0);
+    if (refine) {
+        for (size_t i = 0; i < count; ++i) {
+            const size_t n = size_t(images[i].width) * size_t(images[i].height);
+            const int C = filter_channels(images[i].layer_count, smooth);
+            ab_bytes = std::max(ab_bytes, filter_workspace_bytes(n, C));
+            b.q_offset[i] = planes;
+            planes += (size_t(C) * n + 31) / 32 * 32;
+        }
+    }
+    b.planes = b.ab + align256(ab_bytes);
+    b.total = b.planes + align256(planes * sizeof(double));
+    return b;
+}
+
+}  // namespace
+
+size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth)
+{
+    return batch_layout(images, count, refine, smooth).total;
+}
+
+void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& sets, bool refine, bool opaque,
+               bool smooth, const unblending_hip_params& params, void* workspace, hipStream_t stream,
+               unblending_hip_stats* stats)
+{
+    const size_t count = sets.size();
+    const bool timing = params.timing != 0;
+    const BatchLayout lay = batch_layout(images, count, refine, smooth);
+    char* ws = static_cast<char*>(workspace);
+    auto* counters = reinterpret_cast<unsigned long long*>(ws);
+    ImageDesc* ddesc = reinterpret_cast<ImageDesc*>(ws + lay.desc);
+    double* planes = reinterpret_cast<double*>(ws + lay.planes);
+    if (stats) *stats = unblending_hip_stats{};
+    check(launch_zero(counters, kNumCounters, stream), "zero counters");
+
+    // refinement: filter every image's alphas (and background) into its planes
+    if (refine) {
+        double* ab = reinterpret_cast<double*>(ws + lay.ab);
+        unblending_hip_stats fs{};
+        double filter_ms = 0.0, coeff_ms = 0.0, out_ms = 0.0;
+        int launches = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const unblending_hip_image& im = images[i];
+            const size_t n = size_t(im.width) * size_t(im.height), L = sets[i].host.size();
+            int radius = im.filter_radius;
+            if (radius < 0) radius = 60 * std::min(im.width, im.height) / 1000;  // unblending.cpp:351
+            const double eps = params.filter_epsilon > 0.0 ? params.filter_epsilon : 1e-4;  // :353
+            const int C = filter_channels(L, smooth);
+            std::vector<Channel> in(C);
+            std::vector<double*> q(C);
+            for (size_t l = 0; l < L; ++l)
+                in[l] = im.alpha_planes ? Channel{im.alpha_planes + l * n, 1} : Channel{im.first_pass + 4 * l * n + 3, 4};
+            for (int c = 0; c < (smooth ? 3 : 0); ++c) in[L + c] = Channel{im.first_pass + c, 4};
+            for (int c = 0; c < C; ++c) q[c] = planes + lay.q_offset[i] + size_t(c) * n;
+            run_filter(im.rgb, im.width, im.height, in.data(), q.data(), C, radius, eps, ab, stream,
+                       timing ? &fs : nullptr, timing, params.filter_block, params.filter_rows);
+            filter_ms += fs.filter_ms;
+            coeff_ms += fs.filter_coeff_ms;
+            out_ms += fs.filter_out_ms;
+            launches += fs.filter_launches;
+        }
+        if (stats) {
+            stats->filter_ms = filter_ms;
+            stats->filter_coeff_ms = coeff_ms;
+            stats->filter_out_ms = out_ms;
+            stats->filter_launches = launches;
+        }
+    }
+
+    // descriptors, grouped by layer count (stable)
+    std::vector<size_t> order(count);
+    for (size_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return sets[a].host.size() < sets[b].host.size(); });
+    const int chunk = params.chunk_size > 0 ? params.chunk_size : kDefaultChunk;
+    const SolverParams sp = default_solver(params);
+    struct Group {
+        int L;
+        size_t first, count;
+        long long chunks;
+    };
+    std::vector<Group> groups;
+    long long chunk_base = 0;
+    for (size_t k = 0; k < count; ++k) {
+        const size_t i = order[k];
+        const unblending_hip_image& im = images[i];
+        const LayerSet& set = sets[i];
+        const int L = int(set.host.size());
+        if (groups.empty() || groups.back().L != L) {
+            groups.push_back(Group{L, k, 0, 0});
+            chunk_base = 0;
+        }
+        ImageDesc d{};
+        for (int l = 0; l < L; ++l) d.layers[l] = to_device(set.host[l]);
+        d.rgb = im.rgb;
+        d.n = (long long)im.width * im.height;
+        d.out = im.output;
+        if (refine) {
+            d.first = im.first_pass;
+            d.filtered = planes + lay.q_offset[i];
+        } else {
+            d.alpha_planes = im.alpha_planes;
+        }
+        d.chunk_base = chunk_base;
+        d.L = L;
+        d.opaque = opaque ? 1 : 0;
+        d.smooth = (refine && smooth) ? 1 : 0;
+        d.all_plus = set.all_plus ? 1 : 0;
+        d.all_so = set.all_so ? 1 : 0;
+        check(launch_store_desc(ddesc + k, d, stream), "image descriptor");
+        const long long chunks = (d.n + chunk - 1) / chunk;
+        chunk_base += chunks;
+        groups.back().count += 1;
+        groups.back().chunks += chunks;
+    }
+    if (groups.size() > size_t(kNumCounters - kGroupQueue))
+        throw ExecutionError("too many distinct layer counts in one batch");
+
+    Events es(timing);
+    es.start(stream);
+    for (size_t g = 0; g < groups.size(); ++g) {
+        BatchArgs a{};
+        a.images = ddesc + groups[g].first;
+        a.image_count = int(groups[g].count);
+        a.chunk_size = chunk;
+        a.chunk_count = groups[g].chunks;
+        a.sp = sp;
+        a.lane_state = reinterpret_cast<double*>(ws + lay.lane_state);
+        a.counters = counters;
+        a.queue = kGroupQueue + int(g);
+        check(launch_batch_solve(groups[g].L, refine, a, stream), "batch solve");
+    }
+    es.stop(stream);
+    if (stats) {
+        unsigned long long host_counters[kNumCounters] = {};
+        check(hipMemcpyAsync(host_counters, counters, sizeof host_counters, hipMemcpyDeviceToHost, stream),
+              "counters");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        stats->evals = host_counters[kEvals];
+        stats->iterations = host_counters[kIterations];
+        stats->outer = host_counters[kOuter];
+        stats->pixels = host_counters[kPixels];
+        stats->solve_ms = es.ms();
     }
 }
 

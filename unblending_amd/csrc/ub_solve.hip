@@ -286,8 +286,8 @@ __device__ __forceinline__ double norm(const double* v)
 // target colour; refinement targets = crop + normalisation of the filtered
 // alphas (unblending.cpp:366-389, 311-333) and the cropped smoothed
 // background; x = default or refinement initialisation.
-template <int L, bool REFINE>
-__device__ __forceinline__ void load_pixel(const PixelArgs& args, const DevLayer* lay, long long i, double tgt[3],
+template <int L, bool REFINE, class Src>
+__device__ __forceinline__ void load_pixel(const Src& args, const DevLayer* lay, long long i, double tgt[3],
                                            double ta[L], double bg[3], double* x)
 {
 #pragma unroll
@@ -435,15 +435,21 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 }
 
 // ---------------------------------------------------------------------------
-// Persistent, evaluation-synchronous solve.
+// Persistent, evaluation-synchronous solve over a batch of images.
 //
 // Every trip around the kernel's loop evaluates the objective once for every
 // lane; between evaluations each lane advances its own pixel's state machine
 // (start of an inner solve, line-search trial, search direction, outer
-// update, next pixel).  A lane whose pixel is finished takes the next pixel
-// index from a device counter, so lanes of one wave stay busy however uneven
-// the per-pixel iteration counts are.  The arithmetic is the same sequence,
-// per pixel, as solve_simple_kernel and the host oracle.
+// update, next pixel).  A lane whose pixel is finished takes the next item of
+// the wave's current chunk, and the wave takes the next chunk from a device
+// counter, so lanes stay busy however uneven the per-pixel iteration counts
+// are.  A wave solves one image at a time (the layer table is wave-uniform):
+// when its next chunk belongs to another image, lanes that finish wait until
+// the wave has drained, and the wave then moves on.  A launch holds images
+// of one layer count (the kernel is compiled per layer count, which keeps
+// each one's register allocation its own); one image is a batch of one.  The arithmetic
+// is the same sequence, per pixel, as solve_simple_kernel and the host
+// oracle.
 //
 // Registers hold the current point and the (s, y) pair; the pair's registers
 // hold the trial point and trial gradient between a search direction and its
@@ -456,7 +462,7 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // Only the outer iterate, read and written once per outer iteration, has a
 // per-lane global slot.
 // ---------------------------------------------------------------------------
-enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kIdle = 3 };
+enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kWait = 3 };
 
 // A per-lane vector in LDS, laid out [element][64 lanes].
 struct LaneVec {
@@ -495,34 +501,93 @@ struct PersistentShape {
     static constexpr bool kPixelInLds = REFINE && 2 * N + NC + NCC + (3 + L) <= kLaneBudget;
 };
 
-template <int L, bool REFINE, int M>
-__global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
+template <int L, bool REFINE>
+constexpr int persistent_lds_doubles()
 {
-    static_assert(M == 1, "the persistent kernel keeps one (s, y) pair");
+    using PS = PersistentShape<L, REFINE>;
+    return 2 * PS::N + PS::NC + PS::NCC + PixelConst<L, PS::kPixelInLds>::kCount;
+}
+
+static_assert(persistent_lds_doubles<7, true>() <= PersistentShape<7, true>::kLaneBudget, "LDS budget");
+static_assert(persistent_lds_doubles<8, true>() <= PersistentShape<8, true>::kLaneBudget, "LDS budget");
+static_assert(persistent_lds_doubles<8, false>() <= PersistentShape<8, false>::kLaneBudget, "LDS budget");
+
+__device__ __forceinline__ long long uniform64(unsigned long long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+typedef const __attribute__((address_space(4))) ImageDesc ImageDescC;
+
+__device__ __forceinline__ ImageDescC* as_constant(const ImageDesc* p) { return (ImageDescC*)p; }
+
+// Wave-uniform position in the launch's chunk queue.
+struct WaveQueue {
+    int img;                // image being solved; -1 once the queue is empty
+    long long next, end;    // items of the current chunk not yet handed out
+    int pimg;               // a claimed chunk of another image, started once
+    long long pnext, pend;  // the wave has drained
+    bool pending, exhausted;
+};
+
+struct LaneCounters {
+    unsigned long long evals, iters, outer, pixels;
+};
+
+// Claims the next non-empty chunk of the queue: its image and item range.
+// Returns false when the queue is exhausted.  Wave-uniform.
+__device__ __forceinline__ bool claim_chunk(const BatchArgs& a, int& img, long long& b, long long& e)
+{
+    for (;;) {
+        unsigned long long k = 0;
+        if (threadIdx.x == 0) k = atomicAdd(&a.counters[a.queue], 1ull);
+        const long long kk = uniform64(__shfl(k, 0));
+        if (kk >= a.chunk_count) return false;
+        int i = a.image_count - 1;
+        ImageDescC* images = as_constant(a.images);
+        while (i > 0 && images[i].chunk_base > kk) --i;
+        ImageDescC* d = images + i;
+        const long long lim = d->list ? (long long)*d->list_count : d->n;
+        const long long bb = (kk - d->chunk_base) * (long long)a.chunk_size;
+        const long long ee = (bb + a.chunk_size < lim) ? bb + a.chunk_size : lim;
+        if (bb < ee) {
+            img = i;
+            b = bb;
+            e = ee;
+            return true;
+        }
+    }
+}
+
+// The solve of the wave's current image and of the images that follow it in
+// the queue (all with L layers).  Returns when the queue is exhausted.
+template <int L, bool REFINE>
+__device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, WaveQueue& q, LaneCounters& lc)
+{
     using PS = PersistentShape<L, REFINE>;
     constexpr int N = PS::N;
     constexpr int NC = PS::NC;
     using PC = PixelConst<L, PS::kPixelInLds>;
-    __shared__ double s_g[N][64];    // gradient at the current point
-    __shared__ double s_d[N][64];    // search direction
-    __shared__ double s_lam[NC][64]; // multipliers
-    __shared__ double s_c[PS::NCC][64];  // composite constraints at the current point
-    __shared__ double s_px[PC::kCount][64];
     const int lane = threadIdx.x;
+    // per-lane LDS vectors: gradient at the current point, search direction,
+    // multipliers, composite constraints at the current point, pixel targets
+    const LaneVec gv{smem + lane};
+    const LaneVec dv{smem + N * 64 + lane};
+    const LaneVec lam{smem + 2 * N * 64 + lane};
+    const LaneVec cc{smem + (2 * N + NC) * 64 + lane};
+    PC pc(smem + (2 * N + NC + PS::NCC) * 64 + lane);
+    double bg[3] = {0.0, 0.0, 0.0};  // background colour (smoothed refinement)
     // the outer iterate, touched once per outer iteration: a per-lane global
     // slot, [workgroup][variable][lane]; its address is formed inside the
     // loop (from the opaque zero below) so that the compiler does not keep N
     // precomputed 64-bit addresses live across the loop
-    double* const xo_base = args.lane_state + (long long)blockIdx.x * (N * 64) + lane;
-    const LaneVec lam{&s_lam[0][lane]};
-    const LaneVec cc{&s_c[0][lane]};
-    PC pc(&s_px[0][lane]);
-    double bg[3] = {0.0, 0.0, 0.0};  // background colour (smoothed refinement)
+    double* const xo_base = args.lane_state + (long long)blockIdx.x * (4 * kMaxLayers * 64) + lane;
 
     const SolverParams sp = args.sp;
-    const bool opaque = args.opaque != 0;
-    const bool smooth = REFINE && args.smooth != 0;
     const bool afixed = REFINE && sp.alpha_fixed != 0;
+    bool opaque = false, smooth = false;
 
     // S, Y: the (s, y) pair, and between direction and acceptance the trial
     // point and trial gradient.
@@ -531,11 +596,10 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
     int phase = kNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
     long long pix = 0;
-    unsigned long long c_evals = 0, c_iters = 0, c_outer = 0, c_pixels = 0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         xc[i] = 0.0;
-        s_g[i][lane] = 0.0;
+        gv[i] = 0.0;
         S[i] = 0.0;
         Y[i] = 0.0;
     }
@@ -559,58 +623,96 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         }
     };
 
-    // items: pixels 0..n-1, or the entries of a pixel list
-    const long long items = args.list ? (long long)args.counters[kFallback] : args.n;
-
     for (;;) {
-        // The layer table is uniform: it is read with scalar loads from the
-        // argument segment.  An opaque zero offset keeps those loads inside
-        // the loop (served by the scalar cache) instead of letting them be
-        // hoisted into ~200 loop-long SGPRs that would spill.
+        // The image descriptor and layer table are uniform: read with scalar
+        // loads.  An opaque zero offset keeps those loads inside the loop
+        // (served by the scalar cache) instead of letting them be hoisted into
+        // ~200 loop-long SGPRs that would spill.
         int zero;
         asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
-        const DevLayer* lay = args.layers + zero;
+        // the descriptors do not change during the launch: read through the
+        // constant address space at a wave-uniform index, they are scalar loads
+        // (the table's address is also read from the argument segment at an
+        // opaque offset, so that it is not hoisted into a register that spills)
+        const int img = __builtin_amdgcn_readfirstlane(q.img) + zero;
+        ImageDescC* dd = as_constant((&args.images)[zero]) + img;
+        const DevLayer* lay = (const DevLayer*)dd->layers;
+        opaque = dd->opaque != 0;
+        smooth = REFINE && dd->smooth != 0;
         double* const xo = xo_base + zero;
 
         // ---- next pixels for the lanes that finished theirs ----
-        const unsigned long long needy = __ballot(phase == kNeedPixel);
-        if (needy) {
-            const int leader = __ffsll((long long)needy) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&args.counters[args.queue], (unsigned long long)__popcll(needy));
-            base = __shfl(base, leader);
-            if (phase == kNeedPixel) {
-                const long long item = (long long)(base + (unsigned long long)__popcll(needy & ((1ull << lane) - 1ull)));
-                if (item >= items) {
-                    phase = kIdle;
+        unsigned long long needy = __ballot(phase == kNeedPixel);
+        long long item = -1;
+        while (needy) {
+            if (q.next >= q.end) {
+                if (q.pending || q.exhausted) break;
+                int i2;
+                long long b2, e2;
+                if (!claim_chunk(args, i2, b2, e2)) {
+                    q.exhausted = true;
+                    break;
+                }
+                if (i2 == q.img) {
+                    q.next = b2;
+                    q.end = e2;
                 } else {
-                    pix = args.list ? args.list[item] : item;
-                    double x0[N], C0[NC], tg0[3], ta0[L];
-                    load_pixel<L, REFINE>(args, lay, pix, tg0, ta0, bg, x0);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) pc.tgt[c] = tg0[c];
-#pragma unroll
-                    for (int l = 0; l < L; ++l) pc.ta[l] = ta0[l];
-#pragma unroll
-                    for (int j = 0; j < NC; ++j) lam[j] = 0.0;
-                    rho = sp.rho0;
-                    ocount = 0;
-                    constraints<L, REFINE>(x0, lay, tg0, ta0, C0);
-                    cn_outer = norm<NC>(C0);
-                    count = 0;
-                    gamma = 1.0;
-#pragma unroll
-                    for (int i = 0; i < N; ++i) {
-                        xo[i * 64] = x0[i];
-                        double lo, hi;
-                        bounds(i, lo, hi);
-                        S[i] = clampd(x0[i], lo, hi);
-                    }
-                    phase = kStart;
+                    q.pending = true;
+                    q.pimg = i2;
+                    q.pnext = b2;
+                    q.pend = e2;
+                    break;
                 }
             }
+            const long long avail = q.end - q.next;
+            const long long rank = __popcll(needy & ((1ull << lane) - 1ull));
+            const bool mine = ((needy >> lane) & 1ull) && rank < avail;
+            if (mine) item = q.next + rank;
+            const unsigned long long took = __ballot(mine);
+            q.next += __popcll(took);
+            needy &= ~took;
         }
-        if (__ballot(phase != kIdle) == 0) break;
+        if (item >= 0) {
+            pix = dd->list ? dd->list[item] : item;
+            double x0[N], C0[NC], tg0[3], ta0[L];
+            load_pixel<L, REFINE>(*dd, lay, pix, tg0, ta0, bg, x0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pc.tgt[c] = tg0[c];
+#pragma unroll
+            for (int l = 0; l < L; ++l) pc.ta[l] = ta0[l];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) lam[j] = 0.0;
+            rho = sp.rho0;
+            ocount = 0;
+            constraints<L, REFINE>(x0, lay, tg0, ta0, C0);
+            cn_outer = norm<NC>(C0);
+            count = 0;
+            gamma = 1.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                xo[i * 64] = x0[i];
+                double lo, hi;
+                bounds(i, lo, hi);
+                S[i] = clampd(x0[i], lo, hi);
+            }
+            phase = kStart;
+        } else if (phase == kNeedPixel) {
+            phase = kWait;
+        }
+
+        // ---- a drained wave moves to its pending chunk, or ends ----
+        if (__ballot(phase == kStart || phase == kTrial) == 0) {
+            if (!q.pending) {
+                q.img = -1;
+                return;
+            }
+            q.img = q.pimg;
+            q.next = q.pnext;
+            q.end = q.pend;
+            q.pending = false;
+            phase = kNeedPixel;
+            continue;
+        }
 
         // ---- one evaluation per lane at the trial point S ----
         double Ct[PS::NCC];
@@ -623,7 +725,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 xc[i] = S[i];
-                s_g[i][lane] = Y[i];
+                gv[i] = Y[i];
             }
 #pragma unroll
             for (int j = 0; j < PS::NCC; ++j) cc[j] = Ct[j];
@@ -632,7 +734,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             ++nev;
             double dphi = 0.0;
 #pragma unroll
-            for (int i = 0; i < N; ++i) dphi += s_g[i][lane] * (S[i] - xc[i]);
+            for (int i = 0; i < N; ++i) dphi += gv[i] * (S[i] - xc[i]);
             if (__builtin_isfinite(ft) && ft <= f + 1e-4 * dphi) {
                 double sy = 0.0, yy = 0.0;
                 bool xstop = true;
@@ -640,13 +742,13 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 for (int i = 0; i < N; ++i) {
                     const double xt = S[i], gt = Y[i];
                     const double sv = xt - xc[i];
-                    const double yv = gt - s_g[i][lane];
+                    const double yv = gt - gv[i];
                     sy += sv * yv;
                     yy += yv * yv;
                     xstop = xstop && (__builtin_fabs(sv) < sp.xtol * (__builtin_fabs(xt) + __builtin_fabs(xc[i])) * 0.5 ||
                                       xt == xc[i]);
                     xc[i] = xt;
-                    s_g[i][lane] = gt;
+                    gv[i] = gt;
                     S[i] = sv;
                     Y[i] = yv;
                 }
@@ -683,7 +785,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 for (int i = 0; i < N; ++i) {
                     double blo, bhi;
                     bounds(i, blo, bhi);
-                    S[i] = clampd(xc[i] + t * s_d[i][lane], blo, bhi);
+                    S[i] = clampd(xc[i] + t * dv[i], blo, bhi);
                 }
             }
         }
@@ -696,7 +798,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             double pgmax = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const double gi = s_g[i][lane];
+                const double gi = gv[i];
                 finite = finite && __builtin_isfinite(gi);
                 double lo, hi;
                 bounds(i, lo, hi);
@@ -709,7 +811,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             if (!finite || pgmax == 0.0) {
                 inner_end = true;
             } else {
-                ++c_iters;
+                ++lc.iters;
                 double a = 0.0;
                 if (count > 0) {
                     double sd = 0.0;
@@ -733,13 +835,13 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     q[i] = ((freev >> i) & 1u) ? -q[i] : 0.0;
-                    gd += s_g[i][lane] * q[i];
+                    gd += gv[i] * q[i];
                 }
                 if (!(gd < 0.0)) {
                     count = 0;
                     gamma = 1.0;
 #pragma unroll
-                    for (int i = 0; i < N; ++i) q[i] = ((freev >> i) & 1u) ? -s_g[i][lane] : 0.0;
+                    for (int i = 0; i < N; ++i) q[i] = ((freev >> i) & 1u) ? -gv[i] : 0.0;
                 }
                 t = 1.0;
                 if (count == 0) {
@@ -752,7 +854,7 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 ls = 0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    s_d[i][lane] = q[i];
+                    dv[i] = q[i];
                     double lo, hi;
                     bounds(i, lo, hi);
                     S[i] = clampd(xc[i] + t * q[i], lo, hi);
@@ -762,8 +864,8 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
         }
 
         if (inner_end) {
-            c_evals += (unsigned long long)nev;
-            c_outer += 1;
+            lc.evals += (unsigned long long)nev;
+            lc.outer += 1;
             // the constraint vector at the current point: the composite part
             // kept from its evaluation, the refinement's alpha part alpha_i - t_i
             double Ccur[NC];
@@ -791,20 +893,20 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
                 // warm start: a pixel whose solve stopped at the outer-iteration
                 // cap without meeting the plain solve's tolerances (residual
                 // |C| < eps and step < eps) goes to the fallback launch
-                if (!REFINE && args.fallback && !(unchanged && satisfied)) {
-                    const unsigned long long k = atomicAdd(&args.counters[kFallback], 1ull);
-                    args.fallback[k] = pix;
+                if (!REFINE && dd->fallback && !(unchanged && satisfied)) {
+                    const unsigned long long k = atomicAdd(dd->fallback_count, 1ull);
+                    dd->fallback[k] = pix;
                 }
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
-                    double* px = args.out + 4 * ((long long)l * args.n + pix);
+                    double* px = dd->out + 4 * ((long long)l * dd->n + pix);
                     px[0] = xc[L + 3 * l];
                     px[1] = xc[L + 3 * l + 1];
                     px[2] = xc[L + 3 * l + 2];
                     px[3] = xc[l];
-                    if (args.alpha_planes) args.alpha_planes[(long long)l * args.n + pix] = xc[l];
+                    if (dd->alpha_planes) dd->alpha_planes[(long long)l * dd->n + pix] = xc[l];
                 }
-                ++c_pixels;
+                ++lc.pixels;
                 phase = kNeedPixel;
             } else {
                 ++ocount;
@@ -820,34 +922,53 @@ __global__ void __launch_bounds__(64) solve_persistent_kernel(PixelArgs args)
             }
         }
     }
+}
 
-    for (int off = 32; off > 0; off >>= 1) {
-        c_evals += __shfl_xor(c_evals, off);
-        c_iters += __shfl_xor(c_iters, off);
-        c_outer += __shfl_xor(c_outer, off);
-        c_pixels += __shfl_xor(c_pixels, off);
+template <int L, bool REFINE>
+__global__ void __launch_bounds__(64) solve_batch_kernel(BatchArgs args)
+{
+    __shared__ double smem[persistent_lds_doubles<L, REFINE>() * 64];
+    LaneCounters lc{0, 0, 0, 0};
+    WaveQueue q{};
+    q.img = -1;
+    int i0;
+    long long b0, e0;
+    if (claim_chunk(args, i0, b0, e0)) {
+        q.img = i0;
+        q.next = b0;
+        q.end = e0;
     }
-    if (lane == 0) {
-        atomicAdd(&args.counters[kEvals], c_evals);
-        atomicAdd(&args.counters[kIterations], c_iters);
-        atomicAdd(&args.counters[kOuter], c_outer);
-        atomicAdd(&args.counters[kPixels], c_pixels);
+    // every image of the launch has L layers (the host groups them so)
+    if (q.img >= 0) run_images<L, REFINE>(args, smem, q, lc);
+    for (int off = 32; off > 0; off >>= 1) {
+        lc.evals += __shfl_xor(lc.evals, off);
+        lc.iters += __shfl_xor(lc.iters, off);
+        lc.outer += __shfl_xor(lc.outer, off);
+        lc.pixels += __shfl_xor(lc.pixels, off);
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&args.counters[kEvals], lc.evals);
+        atomicAdd(&args.counters[kIterations], lc.iters);
+        atomicAdd(&args.counters[kOuter], lc.outer);
+        atomicAdd(&args.counters[kPixels], lc.pixels);
     }
 }
 
 template <int L, bool REFINE>
-hipError_t launch_persistent(const PixelArgs& args, hipStream_t stream)
+hipError_t launch_batch(const BatchArgs& args, hipStream_t stream)
 {
-    auto kernel = solve_persistent_kernel<L, REFINE, kLbfgsMemory>;
+    auto kernel = solve_batch_kernel<L, REFINE>;
     int device = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0);
     if (e != hipSuccess) return e;
     long long blocks = (long long)cus * (per_cu > 0 ? per_cu : 1);
-    const long long needed = (args.n + 63) / 64;
-    if (blocks > needed) blocks = needed;
+    // no more workgroups than chunks (each wave takes at least one)
+    if (blocks > args.chunk_count) blocks = args.chunk_count;
+    if (args.max_blocks > 0 && blocks > args.max_blocks) blocks = args.max_blocks;
     if (blocks > kMaxPersistentBlocks) blocks = kMaxPersistentBlocks;
+    if (blocks < 1) return hipSuccess;
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(64), 0, stream, args);
     return hipGetLastError();
 }
@@ -855,7 +976,6 @@ hipError_t launch_persistent(const PixelArgs& args, hipStream_t stream)
 template <int L>
 hipError_t launch_L(bool refine, const PixelArgs& args, hipStream_t stream)
 {
-    if (args.solver == 0) return refine ? launch_persistent<L, true>(args, stream) : launch_persistent<L, false>(args, stream);
     const unsigned blocks = (unsigned)((args.n + 63) / 64);
     if (refine)
         hipLaunchKernelGGL((solve_simple_kernel<L, true, kLbfgsMemory>), dim3(blocks), dim3(64), 0, stream, args);
@@ -865,6 +985,24 @@ hipError_t launch_L(bool refine, const PixelArgs& args, hipStream_t stream)
 }
 
 }  // namespace
+
+// Writes one image descriptor (passed in the argument segment) to device
+// memory, in stream order.
+__global__ void store_desc_kernel(ImageDesc* dst, ImageDesc d)
+{
+    const int i = threadIdx.x;
+    constexpr int kWords = sizeof(ImageDesc) / sizeof(unsigned long long);
+    static_assert(sizeof(ImageDesc) % sizeof(unsigned long long) == 0, "descriptor size");
+    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(&d);
+    unsigned long long* out = reinterpret_cast<unsigned long long*>(dst);
+    for (int k = i; k < kWords; k += blockDim.x) out[k] = src[k];
+}
+
+hipError_t launch_store_desc(ImageDesc* dst, const ImageDesc& d, hipStream_t stream)
+{
+    hipLaunchKernelGGL(store_desc_kernel, dim3(1), dim3(64), 0, stream, dst, d);
+    return hipGetLastError();
+}
 
 __global__ void zero_kernel(unsigned long long* p, int count)
 {
@@ -876,6 +1014,28 @@ hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream)
 {
     hipLaunchKernelGGL(zero_kernel, dim3((count + 63) / 64), dim3(64), 0, stream, p, count);
     return hipGetLastError();
+}
+
+template <int L>
+hipError_t launch_batch_L(bool refine, const BatchArgs& args, hipStream_t stream)
+{
+    return refine ? launch_batch<L, true>(args, stream) : launch_batch<L, false>(args, stream);
+}
+
+hipError_t launch_batch_solve(int L, bool refine, const BatchArgs& args, hipStream_t stream)
+{
+    if (args.chunk_count <= 0) return hipSuccess;
+    switch (L) {
+    case 1: return launch_batch_L<1>(refine, args, stream);
+    case 2: return launch_batch_L<2>(refine, args, stream);
+    case 3: return launch_batch_L<3>(refine, args, stream);
+    case 4: return launch_batch_L<4>(refine, args, stream);
+    case 5: return launch_batch_L<5>(refine, args, stream);
+    case 6: return launch_batch_L<6>(refine, args, stream);
+    case 7: return launch_batch_L<7>(refine, args, stream);
+    case 8: return launch_batch_L<8>(refine, args, stream);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_solve(int L, bool refine, const PixelArgs& args, hipStream_t stream)

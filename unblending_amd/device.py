@@ -12,7 +12,8 @@ from __future__ import annotations
 
 import ctypes
 
-from . import (BACKEND_HIP, HipParams, HipStats, _lib, hip_params, layer_array, options, raise_for_status)
+from . import (BACKEND_HIP, HipImage, HipParams, HipStats, _lib, hip_params, layer_array, options,
+               raise_for_status)
 
 
 def _torch():
@@ -116,3 +117,84 @@ def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stat
         ctypes.byref(stats) if stats is not None else None)
     raise_for_status(st)
     return out
+
+
+class BatchImage:
+    """One image of a batch: rgb (h, w, 3) on the device, its layer infos,
+    its output (L, h, w, 4) (allocated when None) and, for refinement, its
+    first pass (L, h, w, 4), optional alpha planes (L, h, w) and radius."""
+
+    def __init__(self, rgb, layer_infos, out=None, first=None, alpha_planes=None, radius=-1):
+        torch = _torch()
+        h, w, _ = rgb.shape
+        L = len(layer_infos)
+        _check(rgb, (h, w, 3), "rgb")
+        if out is None:
+            out = torch.empty((L, h, w, 4), dtype=torch.float64, device=rgb.device)
+        _check(out, (L, h, w, 4), "out")
+        if first is not None:
+            _check(first, (L, h, w, 4), "first")
+        if alpha_planes is not None:
+            _check(alpha_planes, (L, h, w), "alpha_planes")
+        self.rgb, self.layer_infos, self.out, self.first, self.alpha_planes = rgb, layer_infos, out, first, alpha_planes
+        self.radius = int(radius)
+        self._layers = layer_array(layer_infos)
+
+    def c_struct(self) -> HipImage:
+        h, w, _ = self.rgb.shape
+        im = HipImage()
+        im.rgb = self.rgb.data_ptr()
+        im.width, im.height = w, h
+        im.layers = ctypes.cast(self._layers, ctypes.c_void_p)
+        im.layer_count = len(self.layer_infos)
+        im.first_pass = self.first.data_ptr() if self.first is not None else None
+        im.alpha_planes = self.alpha_planes.data_ptr() if self.alpha_planes is not None else None
+        im.output = self.out.data_ptr()
+        im.filter_radius = self.radius
+        return im
+
+
+def _batch_array(images):
+    arr = (HipImage * len(images))()
+    for i, b in enumerate(images):
+        arr[i] = b.c_struct()
+    return arr
+
+
+def batch_workspace_size(images, refine, smooth=False) -> int:
+    return int(_lib.unblending_hip_batch_workspace_size(_batch_array(images), len(images), int(bool(refine)),
+                                                        int(bool(smooth))))
+
+
+def batch_workspace(images, refine, smooth=False, device=None):
+    torch = _torch()
+    n = max(1, (batch_workspace_size(images, refine, smooth) + 7) // 8)
+    return torch.empty(n, dtype=torch.float64, device=device or images[0].rgb.device)
+
+
+def unmix_batch(images, opaque=True, ws=None, params: HipParams | None = None, stream=None,
+                stats: HipStats | None = None):
+    """First pass of every BatchImage (outputs, and alpha planes when given)."""
+    if ws is None:
+        ws = batch_workspace(images, False)
+    opt = options(BACKEND_HIP, opaque, False)
+    st = _lib.unblending_hip_unmix_batch_device(
+        _batch_array(images), len(images), ctypes.byref(opt), ctypes.byref(params or hip_params()),
+        ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
+        ctypes.byref(stats) if stats is not None else None)
+    raise_for_status(st)
+    return [b.out for b in images]
+
+
+def refine_batch(images, opaque=True, smooth=False, ws=None, params: HipParams | None = None, stream=None,
+                 stats: HipStats | None = None):
+    """Refinement of every BatchImage (each with its first pass)."""
+    if ws is None:
+        ws = batch_workspace(images, True, smooth)
+    opt = options(BACKEND_HIP, opaque, smooth)
+    st = _lib.unblending_hip_refine_batch_device(
+        _batch_array(images), len(images), ctypes.byref(opt), ctypes.byref(params or hip_params()),
+        ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
+        ctypes.byref(stats) if stats is not None else None)
+    raise_for_status(st)
+    return [b.out for b in images]
