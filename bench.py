@@ -76,6 +76,102 @@ def cpu_baseline(wl, rows, threads):
     return rec, first, second
 
 
+def run_batch_bench(args):
+    """64 images of uneven sizes, each with its own 3-7 layer set: both passes
+    of all images as batched persistent launches (one per layer count), the
+    two passes captured once in a HIP graph that each step replays."""
+    import torch
+    import unblending_amd as U
+    from unblending_amd import autotune, device, synth
+
+    t0 = time.perf_counter()
+    wls = [synth.config5_image(j, args.c5_scale) for j in range(args.c5_images)]
+    log("generated %d images in %.1f s" % (len(wls), time.perf_counter() - t0))
+    n = sum(wl.width * wl.height for wl in wls)
+    dev = torch.device("cuda:0")
+    infos = [[U.LayerInfo(U.CompOp(*l.comp), U.BlendMode(l.mode), U.GaussianColorModel(l.mean, l.precision))
+              for l in wl.layers] for wl in wls]
+    rgbs = [torch.from_numpy(wl.rgb).to(dev) for wl in wls]
+    first = [device.BatchImage(rgbs[j], infos[j], radius=wl.radius) for j, wl in enumerate(wls)]
+    second = [device.BatchImage(rgbs[j], infos[j], first=first[j].out, radius=wl.radius) for j, wl in enumerate(wls)]
+    big = max(range(len(wls)), key=lambda j: wls[j].width * wls[j].height)
+    t0 = time.perf_counter()
+    chunk, _ = autotune.tune_chunk(rgbs[big], infos[big], log=log)
+    tune_s = time.perf_counter() - t0
+    params = U.hip_params(eps=1e-4, chunk_size=chunk)
+    ws1 = device.batch_workspace(first, False)
+    ws2 = device.batch_workspace(second, True)
+
+    def step(p=params, s1=None, s2=None):
+        device.unmix_batch(first, ws=ws1, params=p, stats=s1)
+        device.refine_batch(second, ws=ws2, params=p, stats=s2)
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log("warmup step %d done" % i)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log("timed %d graph replays: %.2f ms/step" % (args.steps, ms))
+    s1, s2 = U.HipStats(), U.HipStats()
+    tp = U.hip_params(eps=1e-4, chunk_size=chunk, timing=True)
+    step(tp, s1, s2)
+    torch.cuda.synchronize()
+    Ls = [len(wl.layers) for wl in wls]
+    # FP64 work of the solves: per-image flops depend on L; use the pixel-weighted mean L
+    Lbar = sum(len(wl.layers) * wl.width * wl.height for wl in wls) / n
+    f1 = float(s1.evals) * (277 * Lbar - 168) + float(s1.iterations) * 128 * Lbar
+    f2 = float(s2.evals) * (277 * Lbar - 168) + float(s2.iterations) * 128 * Lbar
+    dom, dflops, dms = (("refinement solve (FP64)", f2, s2.solve_ms) if s2.solve_ms >= s1.solve_ms
+                        else ("first-pass solve (FP64)", f1, s1.solve_ms))
+    achieved = dflops / (dms * 1e-3) / 1e12
+    fbytes = sum(filter_bytes(wl.width * wl.height, len(wl.layers)) for wl in wls)
+    rec = {
+        "metric": "Mpixel/s unmix+refine, plain vs MLP warm-start at equal error; filter HBM GB/s",
+        "value": round(n / (ms * 1e-3) / 1e6, 4),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, unblending_amd/synth.py)",
+        "config": {"workload": "c5 batch: %d images, %.1f Mpixel, sides %d-%d, 3-7 layers" % (
+            len(wls), n / 1e6, min(min(wl.width, wl.height) for wl in wls), max(max(wl.width, wl.height) for wl in wls)),
+            "layers_histogram": {str(L): Ls.count(L) for L in sorted(set(Ls))}, "filter_eps": 1e-4,
+            "filter_radius": "min(w, h) / 64", "warm_start": "none", "graph": "both passes, one HIP graph replay a step",
+            "chunk_size": chunk, "chunk_tune_s": round(tune_s, 2)},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "launch_ms": round(dms, 3), "flops_per_launch": dflops},
+        "filter": {"achieved": round(fbytes / (s2.filter_ms * 1e-3) / 1e9, 1) if s2.filter_ms > 0 else 0.0,
+                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "ms": round(s2.filter_ms, 3), "launches": s2.filter_launches},
+        "solve": {"unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
+                  "unmix_evals_per_px": s1.evals / n, "refine_evals_per_px": s2.evals / n,
+                  "eager_step_ms": round(s1.solve_ms + s2.solve_ms + s2.filter_ms, 3)},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        wl = wls[0]
+        cpu, ofirst, _ = cpu_baseline(wl, args.cpu_rows, threads)
+        g1 = first[0].out[:, :args.cpu_rows].cpu().numpy()
+        cpu["plain_vs_oracle"] = {"first_max": float(np.abs(g1 - ofirst).max())}
+        rec["cpu_baseline"] = cpu
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +190,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the reference point "
                          "for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--c5-images", type=int, default=64)
+    ap.add_argument("--c5-scale", type=float, default=1.0)
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of autotuned ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
@@ -108,8 +207,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": src.numel() * 8}), flush=True)
         return
+    if args.config == 5:
+        run_batch_bench(args)
+        return
     import unblending_amd as U
-    from unblending_amd import device, synth
+    from unblending_amd import autotune, device, synth
 
     size = tuple(int(v) for v in args.size.split("x")) if args.size else None
     t0 = time.perf_counter()
@@ -128,7 +230,15 @@ def main():
     alpha = torch.empty((L, h, w), dtype=torch.float64, device=dev)
     ws1 = device.workspace(w, h, L, False, device=dev)
     ws2 = device.workspace(w, h, L, True, device=dev)
-    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver)
+    t0 = time.perf_counter()
+    if args.no_tune:
+        shape = {"chunk_size": 0, "filter_block": 0, "filter_rows": 0}
+    else:
+        block, rows, _ = autotune.tune_filter(w, h, wl.radius, L, wl.eps, log=log)
+        chunk, _ = autotune.tune_chunk(rgb, infos, log=log)
+        shape = {"chunk_size": chunk, "filter_block": block, "filter_rows": rows}
+    tune_s = time.perf_counter() - t0
+    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape)
 
     def step(p=params, s1=None, s2=None, o=out):
         device.unmix(rgb, infos, True, out=first, alpha_planes=alpha, ws=ws1, params=p, stats=s1)
@@ -158,7 +268,7 @@ def main():
         return s1, s2
 
     ms_plain = timed(params, out, "plain")
-    s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver), out)
+    s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape), out)
     first_plain = first.clone()
 
     warm = None
@@ -186,7 +296,7 @@ def main():
         log("trained %d steps in %.1f s, loss %.3e -> %.3e"
             % (args.train_steps, train_s, float(np.mean(losses[:10])), float(np.mean(losses[-100:]))))
         del images, targets, wst
-        wparams = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, warm_start=model)
+        wparams = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, warm_start=model, **shape)
         ws1 = device.workspace(w, h, L, False, device=dev, warm=True)
         out_warm = torch.empty_like(out)
         ms_warm = timed(wparams, out_warm, "warm")
@@ -232,7 +342,8 @@ def main():
         "config": {"workload": wl.name, "width": w, "height": h, "layers": L,
                    "modes": [synth.MODE_NAMES[l.mode] for l in wl.layers], "filter_radius": wl.radius,
                    "filter_eps": wl.eps, "warm_start": "mlp" if warm else "none",
-                   "solver": "persistent" if args.solver == 0 else "simple"},
+                   "solver": "persistent" if args.solver == 0 else "simple",
+                   "launch_shape": shape, "autotune_s": round(tune_s, 2)},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": None, "launch_ms": round(dms, 3), "flops_per_launch": dflops},

@@ -101,11 +101,13 @@ UNBLENDING_API int32_t unblending_hip_refine_device(
 
 /* Guided filter alone, device pointers: q[c] = filter(p[c]) for `channels`
  * planes p (each [height][width]) with RGB guidance rgb [height][width][3].
- * Exposed for the filter's parity tests and bandwidth measurement. */
+ * block / rows: launch shape as in unblending_hip_params (0: defaults).
+ * Exposed for the filter's parity tests, bandwidth measurement and the
+ * launch-shape autotuner. */
 UNBLENDING_API int32_t unblending_hip_guided_filter_device(
     const double* rgb, int32_t width, int32_t height, const double* const* p, double* const* q,
-    int32_t channels, int32_t radius, double epsilon, void* workspace, size_t workspace_size,
-    void* stream, unblending_hip_stats* stats);
+    int32_t channels, int32_t radius, double epsilon, int32_t block, int32_t rows, void* workspace,
+    size_t workspace_size, void* stream, unblending_hip_stats* stats);
 
 UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_t height,
                                                           int32_t channels);

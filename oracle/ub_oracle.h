@@ -149,6 +149,15 @@ void orc_box(const double* in, int w, int h, int r, int brute, double* out);
  * pixel, row stride in doubles). */
 void orc_guided_filter(const double* p, const double* rgb, size_t stride, int w, int h, int r,
                        double eps, int brute, double* q);
+/* The device filter's definition: the same statistics from exact integer sums
+ * of values quantised to per-quantity grids (see ub_filter.c).  imax / pmax:
+ * the bounds max|I| and max|p| the grids derive from (<= 0: taken from the
+ * inputs; a band of an image passes the whole image's). */
+void orc_guided_filter_exact(const double* p, const double* rgb, size_t stride, int w, int h, int r, double eps,
+                             double imax, double pmax, double* q);
+int orc_qexp(double x);
+int orc_qscale(int e, int r);
+double orc_max_abs(const double* v, size_t n, size_t step);
 int orc_default_radius(int w, int h);
 
 /* ---- passes ---- */
@@ -158,13 +167,18 @@ int orc_unmix(const double* rgb, size_t stride, int w, int h, const orc_layer* l
               int opaque, const orc_settings* settings, int threads, double* out,
               orc_counters* counters);
 /* Filtered, cropped and normalised target alphas [L][h*w] and (smooth) the
- * cropped smoothed background [3][h*w]; radius < 0 selects the reference's. */
+ * cropped smoothed background [3][h*w]; radius < 0 selects the reference's.
+ * filter: ORC_FILTER_SLIDING / _BRUTE (FP64 sums) or _EXACT (the device's
+ * exact-sum filter); bounds (exact filter, may be NULL): max|I| then max|p|
+ * of each filtered plane (L alphas, then 3 background channels). */
+enum { ORC_FILTER_SLIDING = 0, ORC_FILTER_BRUTE = 1, ORC_FILTER_EXACT = 2 };
 int orc_refine_targets(const double* rgb, size_t stride, int w, int h, const orc_layer* layers,
                        int L, const double* first, int opaque, int smooth, int radius,
-                       double eps, int brute, double* targets, double* bg);
+                       double eps, int filter, const double* bounds, double* targets, double* bg);
 int orc_refine(const double* rgb, size_t stride, int w, int h, const orc_layer* layers, int L,
-               const double* first, int opaque, int smooth, int radius, double eps, int brute,
-               const orc_settings* settings, int threads, double* out, orc_counters* counters);
+               const double* first, int opaque, int smooth, int radius, double eps, int filter,
+               const double* bounds, const orc_settings* settings, int threads, double* out,
+               orc_counters* counters);
 /* Recomposite packed layers into an RGBA image [h*w][4]. */
 void orc_recomposite(const double* packed, int w, int h, const orc_layer* layers, int L,
                      double* rgba);

@@ -116,12 +116,18 @@ def lib():
         L.orc_refine_targets.restype = ctypes.c_int
         L.orc_refine_targets.argtypes = [dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, lp,
                                          ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_double, ctypes.c_int, dp, dp]
+                                         ctypes.c_double, ctypes.c_int, dp, dp, dp]
         L.orc_refine.restype = ctypes.c_int
         L.orc_refine.argtypes = [dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, lp, ctypes.c_int,
                                  dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                 ctypes.c_int, ctypes.POINTER(OrcSettings), ctypes.c_int, dp,
+                                 ctypes.c_int, dp, ctypes.POINTER(OrcSettings), ctypes.c_int, dp,
                                  ctypes.POINTER(OrcCounters)]
+        L.orc_guided_filter_exact.argtypes = [dp, dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.orc_qexp.restype = ctypes.c_int
+        L.orc_qexp.argtypes = [ctypes.c_double]
+        L.orc_qscale.restype = ctypes.c_int
+        L.orc_qscale.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_recomposite.argtypes = [dp, ctypes.c_int, ctypes.c_int, lp, ctypes.c_int, dp]
         _lib = L
     return _lib
@@ -298,6 +304,23 @@ def guided_filter(p, rgb, r, eps, brute=False):
     return q
 
 
+def guided_filter_exact(p, rgb, r, eps, imax=0.0, pmax=0.0):
+    """The device's exact-sum guided filter (bounds <= 0: from the inputs)."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float64)
+    h, w = p.shape
+    q = np.zeros_like(p)
+    lib().orc_guided_filter_exact(_dp(p), _dp(rgb), 0, w, h, int(r), float(eps), float(imax), float(pmax), _dp(q))
+    return q
+
+
+FILTERS = {"sliding": 0, "brute": 1, "exact": 2}
+
+
+def _filter_mode(filter, brute):
+    return FILTERS["brute"] if brute else FILTERS[filter]
+
+
 def default_radius(w, h):
     return lib().orc_default_radius(int(w), int(h))
 
@@ -315,21 +338,36 @@ def unmix(rgb, layers, opaque=True, settings=None, threads=0):
     return out, cnt
 
 
-def refine_targets(rgb, layers, first, opaque=True, smooth=False, radius=-1, eps=1e-4, brute=False):
+def refine_targets(rgb, layers, first, opaque=True, smooth=False, radius=-1, eps=1e-4, brute=False, filter="exact",
+                   bounds=None):
+    """filter: 'exact' (the device's exact-sum filter), 'sliding' or 'brute'
+    (FP64 sums); bounds: [max|I|, max|p| of each filtered plane] or None."""
     rgb = np.ascontiguousarray(rgb, dtype=np.float64)
     first = np.ascontiguousarray(first, dtype=np.float64)
     h, w, _ = rgb.shape
     L = len(layers)
     targets = np.zeros((L, h, w))
     bg = np.zeros((3, h, w))
+    b = None if bounds is None else np.ascontiguousarray(bounds, dtype=np.float64)
     lib().orc_refine_targets(_dp(rgb), 0, w, h, layer_array(layers), L, _dp(first), int(bool(opaque)),
-                             int(bool(smooth)), int(radius), float(eps), int(bool(brute)),
-                             _dp(targets), _dp(bg))
+                             int(bool(smooth)), int(radius), float(eps), _filter_mode(filter, brute),
+                             None if b is None else _dp(b), _dp(targets), _dp(bg))
     return targets, (bg if smooth else None)
 
 
+def filter_bounds(rgb, first, smooth=False):
+    """[max|I|, max|alpha_l| for each layer, (max|c| of the background channels)]
+    of an image and its packed first pass: the exact filter's bounds."""
+    rgb = np.asarray(rgb, dtype=np.float64)
+    first = np.asarray(first, dtype=np.float64)
+    b = [float(np.abs(rgb).max())] + [float(np.abs(first[l, ..., 3]).max()) for l in range(first.shape[0])]
+    if smooth:
+        b += [float(np.abs(first[0, ..., c]).max()) for c in range(3)]
+    return np.array(b)
+
+
 def refine(rgb, layers, first, opaque=True, smooth=False, radius=-1, eps=1e-4, brute=False,
-           settings=None, threads=0):
+           settings=None, threads=0, filter="exact", bounds=None):
     rgb = np.ascontiguousarray(rgb, dtype=np.float64)
     first = np.ascontiguousarray(first, dtype=np.float64)
     h, w, _ = rgb.shape
@@ -337,9 +375,11 @@ def refine(rgb, layers, first, opaque=True, smooth=False, radius=-1, eps=1e-4, b
     out = np.zeros((L, h, w, 4))
     cnt = OrcCounters()
     st = settings or default_settings()
+    b = None if bounds is None else np.ascontiguousarray(bounds, dtype=np.float64)
     rc = lib().orc_refine(_dp(rgb), 0, w, h, layer_array(layers), L, _dp(first), int(bool(opaque)),
-                          int(bool(smooth)), int(radius), float(eps), int(bool(brute)), ctypes.byref(st),
-                          int(threads), _dp(out), ctypes.byref(cnt))
+                          int(bool(smooth)), int(radius), float(eps), _filter_mode(filter, brute),
+                          None if b is None else _dp(b), ctypes.byref(st), int(threads), _dp(out),
+                          ctypes.byref(cnt))
     assert rc == 0
     return out, cnt
 

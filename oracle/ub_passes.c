@@ -131,9 +131,19 @@ int orc_unmix(const double* rgb, size_t stride, int w, int h, const orc_layer* l
 
 static double crop(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
 
+static void filter_plane(const double* plane, const double* rgb, size_t stride, int w, int h, int radius,
+                         double eps, int filter, const double* bounds, int index, double* out)
+{
+    if (filter == ORC_FILTER_EXACT)
+        orc_guided_filter_exact(plane, rgb, stride, w, h, radius, eps, bounds ? bounds[0] : 0.0,
+                                bounds ? bounds[1 + index] : 0.0, out);
+    else
+        orc_guided_filter(plane, rgb, stride, w, h, radius, eps, filter == ORC_FILTER_BRUTE, out);
+}
+
 int orc_refine_targets(const double* rgb, size_t stride, int w, int h, const orc_layer* layers,
                        int L, const double* first, int opaque, int smooth, int radius,
-                       double eps, int brute, double* targets, double* bg)
+                       double eps, int filter, const double* bounds, double* targets, double* bg)
 {
     (void)opaque;
     const size_t n = (size_t)w * h;
@@ -149,7 +159,7 @@ int orc_refine_targets(const double* rgb, size_t stride, int w, int h, const orc
     /* unblending.cpp:357-363: filter every alpha, layer 0 included */
     for (int l = 0; l < L; ++l) {
         for (size_t i = 0; i < n; ++i) plane[i] = first[4 * ((size_t)l * n + i) + 3];
-        orc_guided_filter(plane, rgb, stride, w, h, radius, eps, brute, targets + (size_t)l * n);
+        filter_plane(plane, rgb, stride, w, h, radius, eps, filter, bounds, l, targets + (size_t)l * n);
     }
     /* :366-389 crop then normalise */
     for (size_t i = 0; i < n; ++i) {
@@ -170,7 +180,7 @@ int orc_refine_targets(const double* rgb, size_t stride, int w, int h, const orc
     if (smooth && bg) {
         for (int c = 0; c < 3; ++c) {
             for (size_t i = 0; i < n; ++i) plane[i] = first[4 * i + c];
-            orc_guided_filter(plane, rgb, stride, w, h, radius, eps, brute, bg + (size_t)c * n);
+            filter_plane(plane, rgb, stride, w, h, radius, eps, filter, bounds, L + c, bg + (size_t)c * n);
             for (size_t i = 0; i < n; ++i) bg[(size_t)c * n + i] = crop(bg[(size_t)c * n + i]);
         }
     }
@@ -179,8 +189,9 @@ int orc_refine_targets(const double* rgb, size_t stride, int w, int h, const orc
 }
 
 int orc_refine(const double* rgb, size_t stride, int w, int h, const orc_layer* layers, int L,
-               const double* first, int opaque, int smooth, int radius, double eps, int brute,
-               const orc_settings* settings, int threads, double* out, orc_counters* counters)
+               const double* first, int opaque, int smooth, int radius, double eps, int filter,
+               const double* bounds, const orc_settings* settings, int threads, double* out,
+               orc_counters* counters)
 {
     if (L < 1 || L > ORC_MAX_LAYERS) return 1;
     if (smooth && !opaque) return 1; /* unblending.cpp:599-600 */
@@ -192,7 +203,7 @@ int orc_refine(const double* rgb, size_t stride, int w, int h, const orc_layer* 
     const size_t n = (size_t)w * h;
     double* targets = (double*)malloc(sizeof(double) * n * (size_t)L);
     double* bg = smooth ? (double*)malloc(sizeof(double) * n * 3) : NULL;
-    orc_refine_targets(rgb, stride, w, h, layers, L, first, opaque, smooth, radius, eps, brute,
+    orc_refine_targets(rgb, stride, w, h, layers, L, first, opaque, smooth, radius, eps, filter, bounds,
                        targets, bg);
     /* the first pass may alias out: read it before the solve writes */
     double* first_copy = NULL;

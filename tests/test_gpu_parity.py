@@ -1,9 +1,11 @@
 """GPU parity: the HIP passes against the oracle on the same seeded inputs.
 
 The device solve runs the same algorithm in the same FP64 operation order as
-the oracle, so agreement is expected to the last bit; the bar asserted is the
-one the path is judged by: per-pixel alphas and colours within 1e-3 max abs,
-recomposite RMS difference at most 1e-4, guided filter within 1e-6.
+the oracle, and the guided filter takes its box means from exact integer sums
+(the oracle's exact-sum filter), so both passes agree with the oracle to the
+last bit; that is asserted, together with the bar the path is judged by:
+per-pixel alphas and colours within 1e-3 max abs, recomposite RMS difference
+at most 1e-4, guided filter within 1e-6 of the FP64 brute-force filter.
 """
 import numpy as np
 import pytest
@@ -42,6 +44,7 @@ def test_unmix_matches_oracle(oracle, k, size, solver):
     ref, _ = oracle.unmix(wl.rgb, oracle_layers(wl.layers), opaque=True)
     d = _report("unmix c%d" % k, gpu, ref)
     assert d.max() <= ALPHA_COLOR_TOL
+    assert d.max() == 0.0
     rg = oracle.recomposite(gpu, oracle_layers(wl.layers))
     rr = oracle.recomposite(ref, oracle_layers(wl.layers))
     assert np.sqrt(np.mean((rg - rr) ** 2)) <= RECOMPOSITE_RMS_TOL
@@ -60,6 +63,7 @@ def test_refine_matches_oracle(oracle, k, size, solver):
     ref, _ = oracle.refine(wl.rgb, ol, first, opaque=True, radius=wl.radius, eps=wl.eps)
     d = _report("refine c%d" % k, gpu, ref)
     assert d.max() <= ALPHA_COLOR_TOL
+    assert d.max() == 0.0
     rg = oracle.recomposite(gpu, ol)
     rr = oracle.recomposite(ref, ol)
     assert np.sqrt(np.mean((rg - rr) ** 2)) <= RECOMPOSITE_RMS_TOL
@@ -72,13 +76,34 @@ def test_guided_filter_matches_oracle(oracle, w, h, r, eps):
     from unblending_amd import device
     rng = np.random.default_rng(w * 1000 + h + r)
     rgb = rng.uniform(0, 1, (h, w, 3))
-    planes = [rng.uniform(0, 1, (h, w)) for _ in range(3)]
+    planes = [rng.uniform(0, 1, (h, w)) * (0.5 + 0.25 * c) for c in range(4)]  # two launch groups
     print("case filter %dx%d r=%d" % (w, h, r), flush=True)
     rgb_t = torch.from_numpy(rgb).cuda()
     out = device.guided_filter(rgb_t, [torch.from_numpy(p).cuda() for p in planes], r, eps)
     torch.cuda.synchronize()
     for p, q in zip(planes, out):
+        qn = q.cpu().numpy()
         ref = oracle.guided_filter(p, rgb, r, eps, brute=True)
-        err = np.abs(q.cpu().numpy() - ref).max()
-        print("  max|q - q_ref| %.3e" % err, flush=True)
+        err = np.abs(qn - ref).max()
+        print("  max|q - q_brute| %.3e" % err, flush=True)
         assert err <= FILTER_TOL
+        assert np.array_equal(qn, oracle.guided_filter_exact(p, rgb, r, eps))
+
+
+@pytest.mark.parametrize("block,rows", [(256, 7), (512, 0), (1024, 33), (256, 200)])
+def test_guided_filter_launch_shapes(oracle, block, rows):
+    """Every launch shape the autotuner may pick computes the same filter."""
+    import torch
+    from unblending_amd import device
+    w, h, r, eps = 150, 90, 9, 1e-4
+    rng = np.random.default_rng(block + rows)
+    rgb = rng.uniform(0, 1, (h, w, 3))
+    planes = [rng.uniform(0, 1, (h, w)) for _ in range(2)]
+    print("case filter shape block %d rows %d" % (block, rows), flush=True)
+    out = device.guided_filter(torch.from_numpy(rgb).cuda(), [torch.from_numpy(p).cuda() for p in planes], r, eps,
+                               block=block, rows=rows)
+    torch.cuda.synchronize()
+    for p, q in zip(planes, out):
+        ref = oracle.guided_filter(p, rgb, r, eps, brute=True)
+        assert np.abs(q.cpu().numpy() - ref).max() <= FILTER_TOL
+        assert np.array_equal(q.cpu().numpy(), oracle.guided_filter_exact(p, rgb, r, eps))

@@ -93,9 +93,12 @@ def test_refine_targets_source_over_keeps_alphas(oracle):
     ls = [oracle.Layer(0, (0.5, 0.5, 0.5), tuple(np.eye(3).ravel() * 50))] * L
     t, bg = oracle.refine_targets(I, ls, first, radius=2, eps=1e-4)
     for l in range(L):
-        q = np.clip(oracle.guided_filter(np.ascontiguousarray(first[l, ..., 3]), I, 2, 1e-4), 0, 1)
+        q = np.clip(oracle.guided_filter_exact(np.ascontiguousarray(first[l, ..., 3]), I, 2, 1e-4), 0, 1)
         assert np.abs(t[l] - q).max() == 0
     assert bg is None
+    # the FP64-sum filters give the same targets to rounding
+    ts, _ = oracle.refine_targets(I, ls, first, radius=2, eps=1e-4, filter="sliding")
+    assert np.abs(ts - t).max() < 1e-11
 
 
 def test_refine_targets_plus_normalises_and_nonopaque_zeroes(oracle):
@@ -119,5 +122,130 @@ def test_refine_targets_smooth_background(oracle):
     ls = [oracle.Layer(0, (0.5, 0.5, 0.5), tuple(np.eye(3).ravel() * 50))] * L
     _, bg = oracle.refine_targets(I, ls, first, smooth=True, radius=2, eps=1e-4)
     for c in range(3):
-        q = np.clip(oracle.guided_filter(np.ascontiguousarray(first[0, ..., c]), I, 2, 1e-4), 0, 1)
+        q = np.clip(oracle.guided_filter_exact(np.ascontiguousarray(first[0, ..., c]), I, 2, 1e-4), 0, 1)
         assert np.abs(bg[c] - q).max() == 0
+
+
+# ---- the exact-sum filter (the device filter's definition) ----
+
+def _qexp(x):
+    """smallest e with |x| < 2^e (zero: -1022), from the IEEE exponent"""
+    x = abs(float(x))
+    if x == 0.0:
+        return -1022
+    m, e = np.frexp(x)  # x = m 2^e, m in [0.5, 1)
+    return int(e)
+
+
+def _qscale(e, r):
+    n = (2 * r + 1) ** 2
+    return min(62 - e - int(np.ceil(np.log2(n))), 1000)
+
+
+def _qbox_brute(v, r, e):
+    """window means of quantised values, brute-force window sums in Python
+    integers (no sliding, no int64 wrap)"""
+    h, w = v.shape
+    if r == 0:
+        return v.copy()
+    k = _qscale(e, r)
+    q = [[int(np.rint(np.ldexp(v[y, x], k))) for x in range(w)] for y in range(h)]
+    wgt = 1.0 / float((2 * r + 1) ** 2)
+    out = np.zeros_like(v)
+    for y in range(h):
+        for x in range(w):
+            s = 0
+            for dy in range(-r, r + 1):
+                for dx in range(-r, r + 1):
+                    s += q[min(max(y + dy, 0), h - 1)][min(max(x + dx, 0), w - 1)]
+            assert abs(s) < 2 ** 62
+            out[y, x] = np.ldexp(float(s), -k) * wgt
+    return out
+
+
+def _exact_filter_numpy(p, I, r, eps):
+    """Independent restatement of the exact-sum guided filter."""
+    imax, pmax = float(np.abs(I).max()), float(np.abs(p).max())
+    eI, eP = _qexp(imax), _qexp(pmax)
+    eG, eC = max(eI, 2 * eI), max(eP, eI + eP)
+    eA = _qexp(7.0 * imax * pmax / eps)
+    eB = _qexp(pmax + 21.0 * imax * imax * pmax / eps)
+    Ic = [I[..., c] for c in range(3)]
+    mI = [_qbox_brute(Ic[c], r, eG) for c in range(3)]
+    mp = _qbox_brute(p, r, eC)
+    cov = [_qbox_brute(Ic[c] * p, r, eC) - mI[c] * mp for c in range(3)]
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    var = [_qbox_brute(Ic[u] * Ic[v], r, eG) - mI[u] * mI[v] for u, v in pairs]
+    m00, m01, m02 = var[0] + eps, var[1], var[2]
+    m11, m12, m22 = var[3] + eps, var[4], var[5] + eps
+    c00 = m11 * m22 - m12 * m12
+    c01 = m02 * m12 - m01 * m22
+    c02 = m01 * m12 - m02 * m11
+    c11 = m00 * m22 - m02 * m02
+    c12 = m01 * m02 - m00 * m12
+    c22 = m00 * m11 - m01 * m01
+    det = m00 * c00 + m01 * c01 + m02 * c02
+    inv = 1.0 / det
+    a0 = (c00 * inv) * cov[0] + (c01 * inv) * cov[1] + (c02 * inv) * cov[2]
+    a1 = (c01 * inv) * cov[0] + (c11 * inv) * cov[1] + (c12 * inv) * cov[2]
+    a2 = (c02 * inv) * cov[0] + (c12 * inv) * cov[1] + (c22 * inv) * cov[2]
+    b = ((mp - a0 * mI[0]) - a1 * mI[1]) - a2 * mI[2]
+    q = _qbox_brute(b, r, eB)
+    for c, a in enumerate((a0, a1, a2)):
+        q = q + _qbox_brute(a, r, eA) * Ic[c]
+    return q
+
+
+def test_quantisation_exponents(oracle):
+    for x, e in [(1.0, 1), (0.99, 0), (0.5, 0), (0.4999, -1), (2.0, 2), (3.0, 2), (0.0, -1022), (-1.5, 1)]:
+        assert oracle.lib().orc_qexp(x) == e == _qexp(x), x
+    assert oracle.lib().orc_qscale(1, 1) == 62 - 1 - 4  # 9 = (2r+1)^2 <= 2^4
+    assert oracle.lib().orc_qscale(0, 32) == 62 - 13  # 4225 <= 2^13
+    assert oracle.lib().orc_qscale(-1022, 3) == 1000
+
+
+@pytest.mark.parametrize("w,h,r,eps", [(7, 5, 1, 1e-4), (6, 4, 2, 1e-3), (5, 5, 3, 1e-4), (4, 3, 8, 1e-4)])
+def test_exact_filter_matches_independent_restatement(oracle, w, h, r, eps):
+    rng = np.random.default_rng(w * 100 + h * 10 + r)
+    I = rng.uniform(0, 1, (h, w, 3))
+    p = rng.uniform(0, 1, (h, w))
+    assert np.array_equal(oracle.guided_filter_exact(p, I, r, eps), _exact_filter_numpy(p, I, r, eps))
+
+
+@pytest.mark.parametrize("w,h,r,eps", [(23, 17, 3, 1e-4), (40, 9, 12, 1e-3), (16, 16, 5, 1e-4)])
+def test_exact_filter_is_the_guided_filter_to_rounding(oracle, w, h, r, eps):
+    """Quantisation moves the statistics by at most the grid step, far
+    below the filter's 1e-6 bar: the exact and FP64 brute-force filters
+    agree to 1e-9."""
+    rng = np.random.default_rng(w + h + r)
+    I = rng.uniform(0, 1, (h, w, 3))
+    p = rng.uniform(0, 1, (h, w))
+    qe = oracle.guided_filter_exact(p, I, r, eps)
+    qb = oracle.guided_filter(p, I, r, eps, brute=True)
+    assert np.abs(qe - qb).max() < 1e-9
+
+
+def test_exact_filter_invariants(oracle):
+    rng = np.random.default_rng(3)
+    I = rng.uniform(0, 1, (11, 13, 3))
+    p = rng.uniform(0, 1, (11, 13))
+    assert np.array_equal(oracle.guided_filter_exact(p, I, 0, 1e-4), p)  # r = 0: identity
+    c = np.full((11, 13), 0.37)
+    assert np.abs(oracle.guided_filter_exact(c, I, 3, 1e-4) - 0.37).max() < 1e-12
+    # bounds passed explicitly equal to the inputs' give the same result; larger bounds only a coarser grid
+    q0 = oracle.guided_filter_exact(p, I, 3, 1e-4)
+    q1 = oracle.guided_filter_exact(p, I, 3, 1e-4, imax=float(np.abs(I).max()), pmax=float(np.abs(p).max()))
+    assert np.array_equal(q0, q1)
+    q2 = oracle.guided_filter_exact(p, I, 3, 1e-4, imax=4.0, pmax=4.0)
+    assert np.abs(q2 - q0).max() < 1e-8
+
+
+def test_planted_mistake_in_restatement_is_caught(oracle):
+    """A dropped clamp at the left edge (a plausible indexing slip) changes
+    the exact filter and fails the bit-exact comparison."""
+    rng = np.random.default_rng(9)
+    I = rng.uniform(0, 1, (5, 7, 3))
+    p = rng.uniform(0, 1, (5, 7))
+    ref = oracle.guided_filter_exact(p, I, 1, 1e-4)
+    shifted = np.roll(p, 1, axis=1)  # the restatement fed a misaligned plane
+    assert not np.array_equal(ref, _exact_filter_numpy(shifted, I, 1, 1e-4))

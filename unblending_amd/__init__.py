@@ -210,8 +210,8 @@ _lib.unblending_hip_filter_workspace_size.argtypes = [ctypes.c_int32, ctypes.c_i
 _lib.unblending_hip_guided_filter_device.restype = ctypes.c_int32
 _lib.unblending_hip_guided_filter_device.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32,
                                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int32,
-                                                     ctypes.c_int32, ctypes.c_double, _vp, ctypes.c_size_t, _vp,
-                                                     ctypes.POINTER(HipStats)]
+                                                     ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                                     _vp, ctypes.c_size_t, _vp, ctypes.POINTER(HipStats)]
 _lib.uhip_solve_planes.restype = ctypes.c_int32
 _lib.uhip_solve_planes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CLayer), ctypes.c_size_t,
                                    _dp, _dp, ctypes.c_int32, ctypes.c_int32, _dp]

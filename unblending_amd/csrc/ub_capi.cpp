@@ -435,19 +435,22 @@ UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_
 UNBLENDING_API int32_t unblending_hip_guided_filter_device(const double* rgb, int32_t width, int32_t height,
                                                            const double* const* p, double* const* q,
                                                            int32_t channels, int32_t radius, double epsilon,
-                                                           void* workspace, size_t workspace_size, void* stream,
+                                                           int32_t block, int32_t rows, void* workspace,
+                                                           size_t workspace_size, void* stream,
                                                            unblending_hip_stats* stats)
 {
     g_last_error.clear();
     try {
-        if (!rgb || !p || !q || channels <= 0 || width <= 0 || height <= 0 || radius < 0 || !(epsilon > 0.0))
+        if (!rgb || !p || !q || channels <= 0 || width <= 0 || height <= 0 || radius < 0 || !(epsilon > 0.0) ||
+            (block != 0 && block != 256 && block != 512 && block != 1024) || rows < 0 ||
+            (block != 0 && block - 2 * radius < 1))
             return UNBLENDING_INVALID_ARGUMENT;
         if (!workspace || workspace_size < filter_workspace_bytes(size_t(width) * size_t(height), channels))
             return UNBLENDING_BUFFER_TOO_SMALL;
         std::vector<Channel> in(channels);
         for (int c = 0; c < channels; ++c) in[c] = Channel{p[c], 1};
         run_filter(rgb, width, height, in.data(), q, channels, radius, epsilon, static_cast<double*>(workspace),
-                   static_cast<hipStream_t>(stream), stats, true);
+                   static_cast<hipStream_t>(stream), stats, stats != nullptr, block, rows);
         return UNBLENDING_OK;
     } catch (const ExecutionError& e) {
         g_last_error = e.what();

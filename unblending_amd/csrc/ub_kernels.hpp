@@ -105,24 +105,40 @@ struct Channel {
     int stride;
 };
 
-constexpr int kMaxGroup = 8;  // channels filtered by one launch
+constexpr int kMaxGroup = 3;      // channels filtered by one strip launch
+constexpr int kMaxChannels = 16;  // channels of one filter call (L alphas + 3 background)
 
 struct FilterArgs {
     const double* rgb;    // guidance [n][3]
     int w, h, r;
     double eps;
     int groups;           // channels in this launch (<= kMaxGroup)
+    int first_channel;    // index of this launch's first channel in the call (its bounds slot - 1)
     Channel in[kMaxGroup];// pass A inputs
     double* ab;           // pass A output / pass B input: [4 * group][n] (a_r, a_g, a_b, b)
     double* q[kMaxGroup]; // pass B outputs
+    const unsigned long long* bounds;  // max|I| then max|p_c| of the call's channels (bit patterns)
     int block;            // workgroup width (256, 512 or 1024; 0: chosen by radius)
     int rows;             // rows per workgroup (0: chosen by image size)
 };
 
-// Pass A: box statistics and per-pixel coefficients; pass B: box of the
-// coefficients and the output q.  Returns hipErrorInvalidValue when the
-// radius is beyond the strip kernels' reach (r > kMaxFilterRadius).
+// max|I| and max|p_c| of every channel of a filter call: bounds[0] and
+// bounds[1 + c] (bit patterns of non-negative doubles, zeroed beforehand).
+struct BoundsArgs {
+    const double* rgb;
+    int w, h;
+    int channels;
+    Channel in[kMaxChannels];
+    unsigned long long* bounds;
+};
+
+// The exact-sum filter (ub_filter.hip): the bounds pass, then per group of
+// channels pass A (box statistics and per-pixel coefficients) and pass B (box
+// of the coefficients and the output q).  The strip launches return
+// hipErrorInvalidValue when the radius is beyond their reach
+// (r > kMaxFilterRadius).
 constexpr int kMaxFilterRadius = 480;
+hipError_t launch_filter_bounds(const BoundsArgs& args, hipStream_t stream);
 hipError_t launch_filter_coefficients(const FilterArgs& args, hipStream_t stream);
 hipError_t launch_filter_output(const FilterArgs& args, hipStream_t stream);
 

@@ -117,10 +117,14 @@ void require_device()
 
 }  // namespace
 
+// filter workspace: the bounds slots, then one group's coefficient planes
+constexpr size_t kBoundsBytes = 256;
+static_assert(kBoundsBytes >= (1 + kMaxChannels) * sizeof(unsigned long long), "bounds slots");
+
 size_t filter_workspace_bytes(size_t n, int channels)
 {
     const size_t group = size_t(std::min(channels, kMaxGroup));
-    return align256(4 * group * n * sizeof(double));
+    return kBoundsBytes + align256(4 * group * n * sizeof(double));
 }
 
 namespace {
@@ -201,9 +205,28 @@ void run_filter(const double* rgb, int w, int h, const Channel* in, double* cons
 {
     if (radius > kMaxFilterRadius)
         throw ExecutionError("Guided-filter radius is outside the device filter's domain");
+    if (channels > kMaxChannels) throw ExecutionError("too many filtered channels");
     Events ea(timing), eb(timing);
     double coeff_ms = 0.0, out_ms = 0.0;
     int launches = 0;
+    // ab points at the filter workspace: bounds slots, then coefficient planes
+    auto* bounds = reinterpret_cast<unsigned long long*>(ab);
+    double* planes = reinterpret_cast<double*>(reinterpret_cast<char*>(ab) + kBoundsBytes);
+    if (radius > 0) {
+        BoundsArgs ba{};
+        ba.rgb = rgb;
+        ba.w = w;
+        ba.h = h;
+        ba.channels = channels;
+        for (int c = 0; c < channels; ++c) ba.in[c] = in[c];
+        ba.bounds = bounds;
+        ea.start(stream);
+        check(launch_zero(bounds, 1 + kMaxChannels, stream), "guided filter (bounds)");
+        check(launch_filter_bounds(ba, stream), "guided filter (bounds)");
+        ea.stop(stream);
+        launches += 2;
+        if (timing) coeff_ms += ea.ms();
+    }
     for (int g0 = 0; g0 < channels; g0 += kMaxGroup) {
         FilterArgs fa{};
         fa.rgb = rgb;
@@ -212,11 +235,13 @@ void run_filter(const double* rgb, int w, int h, const Channel* in, double* cons
         fa.r = radius;
         fa.eps = eps;
         fa.groups = std::min(kMaxGroup, channels - g0);
+        fa.first_channel = g0;
         for (int j = 0; j < fa.groups; ++j) {
             fa.in[j] = in[g0 + j];
             fa.q[j] = q[g0 + j];
         }
-        fa.ab = ab;
+        fa.ab = planes;
+        fa.bounds = bounds;
         fa.block = block;
         fa.rows = rows;
         ea.start(stream);

@@ -96,8 +96,11 @@ def refine(rgb, first, layer_infos, opaque=True, smooth=False, alpha_planes=None
     return out
 
 
-def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stats: HipStats | None = None):
-    """q[c] = guided filter of planes[c] (each (h, w)) with guidance rgb."""
+def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stats: HipStats | None = None,
+                  block=0, rows=0):
+    """q[c] = guided filter of planes[c] (each (h, w)) with guidance rgb.
+    block / rows: launch shape (0: defaults).  With stats, the call records
+    device times (and synchronises)."""
     torch = _torch()
     h, w, _ = rgb.shape
     _check(rgb, (h, w, 3), "rgb")
@@ -112,7 +115,7 @@ def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stat
     pin = (ctypes.c_void_p * C)(*[p.data_ptr() for p in planes])
     qout = (ctypes.c_void_p * C)(*[q.data_ptr() for q in out])
     st = _lib.unblending_hip_guided_filter_device(
-        ctypes.c_void_p(rgb.data_ptr()), w, h, pin, qout, C, int(radius), float(eps),
+        ctypes.c_void_p(rgb.data_ptr()), w, h, pin, qout, C, int(radius), float(eps), int(block), int(rows),
         ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
         ctypes.byref(stats) if stats is not None else None)
     raise_for_status(st)
