@@ -55,18 +55,38 @@ def mlp_train_flops(batch: int) -> float:
     return 2.0 * batch * (2 * fwd + 32 * 128 + 2 * 128 * 128)
 
 
+def profiled_traffic(w, h, L, refine):
+    """HBM bytes per dispatch of the solve kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or
+    None when there is none for it."""
+    path = os.path.join(ROOT, "profiles", "bench_c%d_%d_hbm_traffic.json" % (3, w))
+    if (w, h, L) != (4096, 4096, 7) or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        prof = json.load(f)
+    key = "void solve_batch_kernel<%d, %s>(BatchArgs)" % (L, "true" if refine else "false")
+    k = prof["kernels"].get(key)
+    if not k:
+        return None, None
+    return k["bytes"], "profiles/" + os.path.basename(path)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, rows, threads):
+def cpu_baseline(wl, rows, threads, bounds=None):
+    """The oracle on rows 0..rows-1 of the image.  bounds: the exact filter's
+    bounds of the whole image (max|I|, max|alpha_l|), so that the band's
+    refinement uses the whole image's quantisation grids."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ub_oracle as O
     band = np.ascontiguousarray(wl.rgb[:rows])
     layers = [O.Layer(l.mode, tuple(l.mean), tuple(np.ravel(l.precision)), l.comp) for l in wl.layers]
     t0 = time.perf_counter()
     first, _ = O.unmix(band, layers, opaque=True, threads=threads)
-    second, _ = O.refine(band, layers, first, opaque=True, radius=wl.radius, eps=wl.eps, threads=threads)
+    second, _ = O.refine(band, layers, first, opaque=True, radius=wl.radius, eps=wl.eps, threads=threads,
+                         bounds=bounds)
     dt = time.perf_counter() - t0
     n = band.shape[0] * band.shape[1]
     rec = {"value": n / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
@@ -165,9 +185,17 @@ def run_batch_bench(args):
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         wl = wls[0]
-        cpu, ofirst, _ = cpu_baseline(wl, args.cpu_rows, threads)
-        g1 = first[0].out[:, :args.cpu_rows].cpu().numpy()
-        cpu["plain_vs_oracle"] = {"first_max": float(np.abs(g1 - ofirst).max())}
+        L0 = len(wl.layers)
+        bounds = np.array([float(rgbs[0].abs().max())] + [float(first[0].out[l, ..., 3].abs().max())
+                                                           for l in range(L0)])
+        rows = min(args.cpu_rows, wl.height)
+        cpu, ofirst, osecond = cpu_baseline(wl, rows, threads, bounds)
+        keep = max(0, rows - 2 * wl.radius - 2) if rows < wl.height else rows
+        g1 = first[0].out[:, :rows].cpu().numpy()
+        g2 = second[0].out[:, :keep].cpu().numpy()
+        cpu["plain_vs_oracle"] = {"image": wl.name, "first_max": float(np.abs(g1 - ofirst).max()),
+                                  "refined_max": float(np.abs(g2 - osecond[:, :keep]).max()) if keep else None,
+                                  "refined_rows": keep}
         rec["cpu_baseline"] = cpu
     print(json.dumps(rec), flush=True)
 
@@ -179,7 +207,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--size", type=str, default="")
-    ap.add_argument("--cpu-rows", type=int, default=96)
+    ap.add_argument("--cpu-rows", type=int, default=640)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solver", type=int, default=0)
     ap.add_argument("--no-warm", action="store_true")
@@ -323,6 +351,7 @@ def main():
     else:
         dom, dflops, dms = "first-pass solve (FP64)", solve_flops(k1.evals, k1.iterations, L), k1.solve_ms
     achieved = dflops / (dms * 1e-3) / 1e12
+    traffic, traffic_source = profiled_traffic(w, h, L, "refinement" in dom)
     ms = warm["ms_per_step"] if warm else ms_plain
     value = n / (ms * 1e-3) / 1e6
 
@@ -346,7 +375,8 @@ def main():
                    "launch_shape": shape, "autotune_s": round(tune_s, 2)},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "launch_ms": round(dms, 3), "flops_per_launch": dflops},
+                     "traffic": traffic, "traffic_source": traffic_source, "launch_ms": round(dms, 3),
+                     "flops_per_launch": dflops},
         "filter": {"achieved": round(fgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(fgbs / HBM_PEAK_GBS, 4), "ms": round(s2.filter_ms, 3),
                    "coeff_ms": round(s2.filter_coeff_ms, 3), "out_ms": round(s2.filter_out_ms, 3),
@@ -381,14 +411,17 @@ def main():
         }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu, ofirst, osecond = cpu_baseline(wl, args.cpu_rows, threads)
+        bounds = np.array([float(rgb.abs().max())] + [float(first_plain[l, ..., 3].abs().max()) for l in range(L)])
+        cpu, ofirst, osecond = cpu_baseline(wl, args.cpu_rows, threads, bounds)
         rows = args.cpu_rows
         keep = max(0, rows - 2 * wl.radius - 2)  # refinement rows unaffected by the band's lower edge
         g1 = first_plain[:, :rows].cpu().numpy()
         g2 = out[:, :keep].cpu().numpy()
         cpu["plain_vs_oracle"] = {"first_max": float(np.abs(g1 - ofirst).max()),
                                   "refined_max": float(np.abs(g2 - osecond[:, :keep]).max()) if keep else None,
-                                  "refined_rows": keep}
+                                  "refined_rows": keep,
+                                  "bit_identical": bool(np.array_equal(g1, ofirst) and
+                                                        np.array_equal(g2, osecond[:, :keep]))}
         if warm:
             gw1 = first[:, :rows].cpu().numpy()
             gw2 = out_warm[:, :keep].cpu().numpy()

@@ -259,14 +259,26 @@ struct PassB {
     }
 };
 
-// inclusive prefix sum of a 64-bit integer across the 64 lanes of a wave
-__device__ __forceinline__ long long wave_scan(long long v, int lane)
+// v of the lane the DPP control selects (0 where it selects none), both halves
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ long long dpp64(long long v)
 {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const long long u = __shfl_up(v, off);
-        if (lane >= off) v += u;
-    }
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v & 0xffffffffll), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROW_MASK, 0xf, false);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// inclusive prefix sum of a 64-bit integer across the 64 lanes of a wave:
+// shifts by 1, 2, 4, 8 within rows of 16 lanes, then the row broadcasts of
+// lanes 15 and 31 (GFX9 DPP)
+__device__ __forceinline__ long long wave_scan(long long v, int)
+{
+    v += dpp64<0x111, 0xf>(v);  // row_shr:1
+    v += dpp64<0x112, 0xf>(v);  // row_shr:2
+    v += dpp64<0x114, 0xf>(v);  // row_shr:4
+    v += dpp64<0x118, 0xf>(v);  // row_shr:8
+    v += dpp64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
