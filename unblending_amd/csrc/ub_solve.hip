@@ -462,7 +462,12 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // Only the outer iterate, read and written once per outer iteration, has a
 // per-lane global slot.
 // ---------------------------------------------------------------------------
-enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kWait = 3 };
+enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kWait = 3, kOuter = 4 };
+
+// Lanes waiting for a pixel, or for their outer update, are served together
+// once at least this many wait (or nothing else is left to do).
+constexpr int kFetchBatch = 8;
+constexpr int kOuterBatch = 8;
 
 // A per-lane vector in LDS, laid out [element][64 lanes].
 struct LaneVec {
@@ -642,7 +647,12 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         double* const xo = xo_base + zero;
 
         // ---- next pixels for the lanes that finished theirs ----
+        // (handed out in batches of at least kFetchBatch lanes while others are
+        // still solving: the start of a pixel is a long stretch of code that the
+        // whole wave executes whenever any lane needs it)
         unsigned long long needy = __ballot(phase == kNeedPixel);
+        if (__popcll(needy) < kFetchBatch && __ballot(phase == kStart || phase == kTrial || phase == kOuter) != 0)
+            needy = 0;
         long long item = -1;
         while (needy) {
             if (q.next >= q.end) {
@@ -696,12 +706,12 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 S[i] = clampd(x0[i], lo, hi);
             }
             phase = kStart;
-        } else if (phase == kNeedPixel) {
+        } else if (phase == kNeedPixel && (q.pending || q.exhausted)) {
             phase = kWait;
         }
 
         // ---- a drained wave moves to its pending chunk, or ends ----
-        if (__ballot(phase == kStart || phase == kTrial) == 0) {
+        if (__ballot(phase == kStart || phase == kTrial || phase == kOuter) == 0) {
             if (!q.pending) {
                 q.img = -1;
                 return;
@@ -863,7 +873,11 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             }
         }
 
-        if (inner_end) {
+        if (inner_end) phase = kOuter;
+        const unsigned long long outer_lanes = __ballot(phase == kOuter);
+        const bool run_outer = __popcll(outer_lanes) >= kOuterBatch ||
+                               __ballot(phase == kStart || phase == kTrial) == 0;
+        if (run_outer && phase == kOuter) {
             lc.evals += (unsigned long long)nev;
             lc.outer += 1;
             // the constraint vector at the current point: the composite part
