@@ -71,6 +71,25 @@ def profiled_traffic(w, h, L, refine):
     return k["bytes"], "profiles/" + os.path.basename(path)
 
 
+def first_pass_quality(packed, rgb, layers):
+    """Objective terms of a packed first pass (L, h, w, 4) against its image:
+    the energy sum_l alpha_l (c_l - mu_l)^T P_l (c_l - mu_l) + 0.01 sum_l alpha_l
+    and the constraint residual |(composite RGB - target, composite alpha - 1)|,
+    the quantities the solve trades off (per-pixel mean; residual p99, max)."""
+    from unblending_amd import synth
+    a = packed[..., 3]
+    c = packed[..., :3]
+    energy = np.zeros(a.shape[1:])
+    for l, m in enumerate(layers):
+        dc = c[l] - m.mean
+        energy += a[l] * np.einsum("...i,ij,...j->...", dc, m.precision, dc)
+    energy += 0.01 * a.sum(0)
+    col, alpha = synth.composite(a, c, layers)
+    res = np.sqrt(((col - rgb) ** 2).sum(-1) + (alpha - 1.0) ** 2)
+    return {"energy_mean": float(energy.mean()), "residual_mean": float(res.mean()),
+            "residual_p99": float(np.percentile(res, 99)), "residual_max": float(res.max())}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -332,7 +351,11 @@ def main():
         d1 = (first - first_plain).abs()
         d2 = (out_warm - out).abs()
         per_train = train_s / max(args.train_steps, 1)
+        band = min(h, 512)
+        qp = first_pass_quality(first_plain[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
+        qw = first_pass_quality(first[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
         warm = {"ms_per_step": ms_warm, "s1": w1, "s2": w2, "train_s": train_s, "label_s": label_s,
+                "quality": {"rows": band, "plain": qp, "warm": qw},
                 "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-100:])),
                 "train_tflops": mlp_train_flops(4096) / per_train / 1e12 if per_train > 0 else 0.0,
                 "first_max": float(d1.max()), "first_mean": float(d1.mean()),
@@ -403,6 +426,7 @@ def main():
                                     "frac_gt_1e-3": warm["first_frac_gt_1e-3"]},
             "vs_plain_refined": {"max": warm["refined_max"], "mean": warm["refined_mean"],
                                  "frac_gt_1e-3": warm["refined_frac_gt_1e-3"]},
+            "first_pass_quality": warm["quality"],
             "training": {"images": args.train_images, "size": args.train_size, "steps": args.train_steps,
                          "batch": 4096, "lr": args.train_lr, "label_s": round(warm["label_s"], 3),
                          "train_s": round(warm["train_s"], 3), "loss_first": warm["loss_first"],

@@ -41,13 +41,14 @@ typedef struct unblending_hip_params {
     int32_t alpha_fixed;      /* refinement: 1 = alphas fixed at the filtered targets by bounds */
     int32_t solver;           /* 0 = default kernel, 1 = simple one-pixel-per-lane kernel */
     int32_t timing;           /* 1 = record per-kernel device time in the stats (synchronises) */
-    int32_t warm_start;       /* first pass: 1 = start from the model's prediction (mlp), re-solve
-                                 from the default initialisation where that misses the tolerances */
+    int32_t warm_start;       /* first pass: 1 = start from the model's prediction (mlp); a pixel whose
+                                 warm-started solve has not met the tolerances after warm_outer_cap + 1
+                                 outer iterations is solved again from the default initialisation */
     const void* mlp;          /* device buffer of a trained model (unblending_hip_mlp_*) */
     int32_t chunk_size;       /* pixels a wave takes from the solve queue at once; 0: default */
     int32_t filter_block;     /* guided filter workgroup width (256, 512, 1024); 0: chosen by radius */
     int32_t filter_rows;      /* guided filter rows per workgroup; 0: chosen by image size */
-    int32_t reserved2;
+    int32_t warm_outer_cap;   /* see warm_start; 0: 8 */
 } unblending_hip_params;
 
 #define UNBLENDING_HIP_PARAMS_INIT \
@@ -66,7 +67,7 @@ typedef struct unblending_hip_stats {
     int32_t filter_channels;
     int32_t filter_launches;
     int32_t reserved;
-    uint64_t fallback_pixels; /* warm start: pixels re-solved from the default initialisation */
+    uint64_t fallback_pixels; /* warm start: pixels solved again from the default initialisation */
     double predict_ms;        /* warm start: device time of the model's prediction (timing = 1) */
 } unblending_hip_stats;
 

@@ -151,7 +151,7 @@ class HipParams(ctypes.Structure):
                 ("filter_epsilon", ctypes.c_double), ("alpha_fixed", ctypes.c_int32),
                 ("solver", ctypes.c_int32), ("timing", ctypes.c_int32), ("warm_start", ctypes.c_int32),
                 ("mlp", ctypes.c_void_p), ("chunk_size", ctypes.c_int32), ("filter_block", ctypes.c_int32),
-                ("filter_rows", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
+                ("filter_rows", ctypes.c_int32), ("warm_outer_cap", ctypes.c_int32)]
 
 
 class HipImage(ctypes.Structure):
@@ -285,7 +285,7 @@ def options(backend=BACKEND_AUTO, opaque=True, smooth=False, target_concurrency=
 
 
 def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, warm_start=None, chunk_size=0,
-               filter_block=0, filter_rows=0) -> HipParams:
+               filter_block=0, filter_rows=0, warm_outer_cap=0) -> HipParams:
     """warm_start: a trained ``mlp.WarmStartModel`` (first pass only), or None.
     chunk_size / filter_block / filter_rows: launch shapes (0: defaults; see
     ``unblending_amd.autotune``)."""
@@ -301,6 +301,7 @@ def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, wa
     p.chunk_size = int(chunk_size)
     p.filter_block = int(filter_block)
     p.filter_rows = int(filter_rows)
+    p.warm_outer_cap = int(warm_outer_cap)
     return p
 
 

@@ -22,13 +22,14 @@ struct SolverParams {
     int max_count;   // 20: the outer loop stops when count > max_count
     int max_ls;      // 20
     int alpha_fixed; // refinement alphas fixed by bounds instead of constraints
+    int warm_max_count;  // warm-started attempt: outer cap before the pixel is solved from the default start
     double ftol, xtol, eps, gamma, beta, rho0;
 };
 
 // Per-pass work counters accumulated by the solve kernels.
-// kQueue / kQueue2 are the persistent solve's next-item counters (first
-// launch, fallback launch); kFallback counts the pixels a warm-started solve
-// hands to the fallback launch.
+// kQueue / kQueue2 are the persistent solve's next-item counters; kFallback
+// counts the pixels whose warm-started attempt missed the tolerances and
+// that were solved again from the default start.
 enum Counter : int {
     kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6,
     kGroupQueue = 8,  // batches: the queue of the launch of the g-th layer count is kGroupQueue + g
@@ -48,11 +49,8 @@ struct PixelArgs {
     double* alpha_planes;  // optional copy of the solved alphas [L][n]
     double* lane_state;    // persistent solve: kLaneStateDoubles of per-lane scratch
     unsigned long long* counters;
-    // first pass only: warm start and fallback
+    // first pass only: warm start
     const double* warm;    // predicted first pass, packed [L][n][RGBA], or nullptr
-    long long* fallback;   // warm start: pixels to re-solve from the default initialisation
-    const long long* list; // pixel list to solve instead of 0..n-1 (its length in counters[kFallback])
-    int queue;             // counter slot of the next-item queue
 };
 
 // One-pixel-per-lane solve of one image (PixelArgs::solver = 1).
@@ -69,8 +67,7 @@ struct ImageDesc {
     double* out;             // packed [L][n][RGBA]
     double* alpha_planes;    // optional copy of the solved alphas [L][n]
     const double* warm;      // first pass: predicted first pass (warm start) or nullptr
-    long long* fallback;     // warm start: pixels to re-solve from the default initialisation
-    unsigned long long* fallback_count;
+    unsigned long long* fallback_count;  // warm start: pixels solved again from the default start
     const long long* list;   // solve list[0 .. *list_count) instead of pixels 0 .. n-1
     const unsigned long long* list_count;
     long long n;             // pixels

@@ -89,6 +89,9 @@ struct PatchRows {
 //
 // An epilogue returns a per-element value that is summed over the block and
 // added to *epi.sum() with one atomic per wave when Epi::kReduce is set.
+//
+// With gridDim.z > 1 the reduction is split: workgroup z covers k in
+// [z K, (z + 1) K) and its epilogue writes a partial product (EpiPartial).
 template <class ALoad, class Epi>
 __global__ void __launch_bounds__(256) gemm_kernel(ALoad a, DenseRows bt, long long M, int N, int K, Epi epi)
 {
@@ -98,7 +101,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(ALoad a, DenseRows bt, long l
     if (m0 >= M || n0 >= N) return;
     const int half = lane >> 5, r32 = lane & 31;
     f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    const int kbeg = blockIdx.z * K;
+    for (int k0 = kbeg; k0 < kbeg + K; k0 += 16) {
         const bf16x8 av = a.load(m0 + r32, k0 + 8 * half);
         const bf16x8 bv = bt.load(n0 + r32, k0 + 8 * half);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
@@ -205,18 +209,30 @@ struct EpiBackHidden {
     }
 };
 
-// Weight gradient dW[in][out] (FP32, layout of the master weights).
-struct EpiStore {
-    float* g;
+// One split of a split reduction: partial[z][m][n].
+struct EpiPartial {
+    float* partial;
+    long long M;
     int N;
     static constexpr bool kReduce = false;
     __device__ float* sum() const { return nullptr; }
     __device__ __forceinline__ float operator()(long long m, int n, float v) const
     {
-        g[m * N + n] = v;
+        partial[((long long)blockIdx.z * M + m) * N + n] = v;
         return 0.0f;
     }
 };
+
+// out[m][n] = sum over z, in order, of partial[z][m][n]
+__global__ void split_sum_kernel(const float* partial, int splits, long long M, int N, float* out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * N) return;
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z) s += partial[(long long)z * M * N + i];
+    out[i] = s;
+}
+
 
 // Bias gradient: column sums of dz [rows][N].
 __global__ void colsum_kernel(const float* dz, long long rows, int N, float* out)
@@ -296,6 +312,24 @@ hipError_t gemm(const ALoad& a, const __bf16* bt, long long M, int N, int K, con
     return hipGetLastError();
 }
 
+// Weight gradient g[M][N] = A . Bt^T with the batch (K) split over
+// kGradSplits workgroups along z, the partials summed in a fixed order (the
+// result does not depend on scheduling).  A has M rows, Bt N rows, both K long.
+hipError_t gemm_weight_grad(const __bf16* a, const __bf16* bt, int M, int N, int K, float* partial, float* g,
+                            hipStream_t s)
+{
+    int splits = kGradSplits;
+    while (splits > 1 && (K % (16 * splits)) != 0) splits >>= 1;
+    const int ks = K / splits;
+    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)splits);
+    hipLaunchKernelGGL((gemm_kernel<DenseRows, EpiPartial>), grid, dim3(256), 0, s, DenseRows{a, K}, DenseRows{bt, K},
+                       (long long)M, N, ks, EpiPartial{partial, M, N});
+    const long long mn = (long long)M * N;
+    hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, partial, splits,
+                       (long long)M, N, g);
+    return hipGetLastError();
+}
+
 }  // namespace mlp
 
 using namespace mlp;
@@ -330,8 +364,9 @@ hipError_t mlp_step(const Model& md, const TrainBatch& tb, const TrainBuffers& b
                   EpiOutputTrain{md.w + kOffB[3], tb.targets, buf.idx, B, tb.outputs, tb.layers, tb.w, tb.h,
                                  buf.dz[3], buf.dzb[3], buf.dzt[3], loss}, s)) != hipSuccess) return e;
     // backward: weight gradients, then the gradient flowing into the layer below
-    if ((e = gemm(DenseRows{buf.ht[2], B}, buf.dzt[3], kHidden, kOutPad, (int)B,
-                  EpiStore{buf.grad + kOffW[3], kOutPad}, s)) != hipSuccess) return e;
+    if ((e = gemm_weight_grad(buf.ht[2], buf.dzt[3], kHidden, kOutPad, (int)B, buf.partial, buf.grad + kOffW[3],
+                              s)) != hipSuccess)
+        return e;
     for (int l = 2; l >= 0; --l) {
         const int nxt = l + 1;
         const int kin = (nxt == 3) ? kOutPad : kHidden;
@@ -339,13 +374,15 @@ hipError_t mlp_step(const Model& md, const TrainBatch& tb, const TrainBuffers& b
                       EpiBackHidden{buf.z[l], buf.dz[l], buf.dzb[l], buf.dzt[l], B, kHidden}, s)) != hipSuccess)
             return e;
         if (l > 0) {
-            if ((e = gemm(DenseRows{buf.ht[l - 1], B}, buf.dzt[l], kHidden, kHidden, (int)B,
-                          EpiStore{buf.grad + kOffW[l], kHidden}, s)) != hipSuccess) return e;
+            if ((e = gemm_weight_grad(buf.ht[l - 1], buf.dzt[l], kHidden, kHidden, (int)B, buf.partial,
+                                      buf.grad + kOffW[l], s)) != hipSuccess)
+                return e;
         } else {
             hipLaunchKernelGGL(patch_transpose_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, pr, B,
                                buf.xt);
-            if ((e = gemm(DenseRows{buf.xt, B}, buf.dzt[0], kInPad, kHidden, (int)B,
-                          EpiStore{buf.grad + kOffW[0], kHidden}, s)) != hipSuccess) return e;
+            if ((e = gemm_weight_grad(buf.xt, buf.dzt[0], kInPad, kHidden, (int)B, buf.partial, buf.grad + kOffW[0],
+                                      s)) != hipSuccess)
+                return e;
         }
     }
     for (int l = 0; l < 4; ++l) {

@@ -100,7 +100,10 @@ struct TrainBuffers {
     __bf16* xt;
     float* grad;
     float* loss;  // kLossSlots per-step losses
+    float* partial;  // split weight-gradient products [kGradSplits][128][128]
 };
+
+constexpr int kGradSplits = 32;  // workgroups along the batch of a weight-gradient product
 
 constexpr int kLossSlots = 1024;
 

@@ -22,11 +22,10 @@ void check(hipError_t e, const char* what)
     if (e != hipSuccess) throw ExecutionError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// workspace header: counters at 0, then the image descriptors of the solve
-// launch and of the warm start's fallback launch
+// workspace header: counters at 0, then the image descriptor of the solve
 constexpr size_t kDescOffset = 256;
 constexpr size_t kDescStride = (sizeof(ImageDesc) + 255) / 256 * 256;
-constexpr size_t kHeader = kDescOffset + 2 * kDescStride;
+constexpr size_t kHeader = kDescOffset + kDescStride;
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
@@ -44,9 +43,12 @@ DevLayer to_device(const unblending_layer& l)
     return d;
 }
 
+constexpr int kDefaultWarmOuterCap = 8;
+
 SolverParams default_solver(const unblending_hip_params& p)
 {
     SolverParams s;
+    s.warm_max_count = p.warm_outer_cap > 0 ? p.warm_outer_cap : kDefaultWarmOuterCap;
     s.max_evals = 1000;   // unblending.cpp:249
     s.max_count = 20;     // :246
     s.max_ls = 20;
@@ -176,6 +178,7 @@ mlp::TrainBuffers mlp_train_view(void* ws, int batch)
     }
     b.xt = carve<__bf16>(cur, B * kInPad);
     b.grad = carve<float>(cur, kParamsPadded);
+    b.partial = carve<float>(cur, size_t(mlp::kGradSplits) * kHidden * kHidden);
     b.loss = carve<float>(cur, mlp::kLossSlots);
     return b;
 }
@@ -190,7 +193,7 @@ size_t mlp_train_bytes(int batch)
 size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool warm)
 {
     size_t bytes = kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
-    if (warm && !refine) bytes += mlp_predict_bytes(n) + align256(n * sizeof(long long));
+    if (warm && !refine) bytes += mlp_predict_bytes(n);
     if (refine) {
         const int C = filter_channels(layers, smooth);
         bytes += filter_workspace_bytes(n, C);
@@ -301,7 +304,6 @@ void run_device(const DeviceJob& job)
         if (4 * L > size_t(mlp::kOutPad)) throw ExecutionError("The warm-start model predicts at most 8 layers");
         char* wcur = ws + kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
         const mlp::PredictBuffers pb = mlp_predict_view(wcur, n);
-        args.fallback = reinterpret_cast<long long*>(wcur + mlp_predict_bytes(n));
         const mlp::Model md = mlp::model_view(const_cast<void*>(job.params.mlp));
         Events ep(timing);
         ep.start(job.stream);
@@ -312,7 +314,6 @@ void run_device(const DeviceJob& job)
         predict_ms = ep.ms();
         args.warm = job.out;
     }
-    args.queue = kQueue;
     if (refine) {
         int radius = job.params.filter_radius;
         if (radius < 0) radius = 60 * std::min(job.w, job.h) / 1000;  // unblending.cpp:351
@@ -345,7 +346,6 @@ void run_device(const DeviceJob& job)
         desc.out = args.out;
         desc.alpha_planes = args.alpha_planes;
         desc.warm = args.warm;
-        desc.fallback = args.fallback;
         desc.fallback_count = counters + kFallback;
         desc.n = args.n;
         desc.L = int(L);
@@ -368,22 +368,6 @@ void run_device(const DeviceJob& job)
     es.start(job.stream);
     if (args.solver == 0) check(launch_batch_solve(int(L), refine, batch, job.stream), "solve");
     else check(launch_solve(int(L), refine, args, job.stream), "solve");
-    if (warm) {
-        // re-solve, from the default initialisation, the pixels whose
-        // warm-started solve missed the tolerances
-        ImageDesc fb = desc;
-        fb.warm = nullptr;
-        fb.fallback = nullptr;
-        fb.fallback_count = nullptr;
-        fb.list = args.fallback;
-        fb.list_count = counters + kFallback;
-        ImageDesc* dfb = reinterpret_cast<ImageDesc*>(ws + kDescOffset + kDescStride);
-        check(launch_store_desc(dfb, fb, job.stream), "image descriptor");
-        BatchArgs fbatch = batch;
-        fbatch.images = dfb;
-        fbatch.queue = kQueue2;
-        check(launch_batch_solve(int(L), false, fbatch, job.stream), "fallback solve");
-    }
     es.stop(job.stream);
     if (job.stats) {
         unsigned long long host_counters[kNumCounters] = {};

@@ -288,7 +288,7 @@ __device__ __forceinline__ double norm(const double* v)
 // background; x = default or refinement initialisation.
 template <int L, bool REFINE, class Src>
 __device__ __forceinline__ void load_pixel(const Src& args, const DevLayer* lay, long long i, double tgt[3],
-                                           double ta[L], double bg[3], double* x)
+                                           double ta[L], double bg[3], double* x, bool use_warm = true)
 {
 #pragma unroll
     for (int c = 0; c < 3; ++c) tgt[c] = args.rgb[3 * i + c];
@@ -324,7 +324,7 @@ __device__ __forceinline__ void load_pixel(const Src& args, const DevLayer* lay,
 #pragma unroll
             for (int c = 0; c < 3; ++c) x[L + 3 * l + c] = px[c];
         }
-    } else if (args.warm) {
+    } else if (args.warm && use_warm) {
         // warm start: the predicted first pass
 #pragma unroll
         for (int l = 0; l < L; ++l) {
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // Only the outer iterate, read and written once per outer iteration, has a
 // per-lane global slot.
 // ---------------------------------------------------------------------------
-enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kWait = 3, kOuter = 4 };
+enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kWait = 3, kOuter = 4, kRetry = 5 };
 
 // Lanes waiting for a pixel, or for their outer update, are served together
 // once at least this many wait (or nothing else is left to do).
@@ -600,6 +600,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     double rh = 0.0;
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
     int phase = kNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
+    bool warm_try = false;  // this pixel's solve started from the warm-start prediction
     long long pix = 0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -651,8 +652,10 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         // still solving: the start of a pixel is a long stretch of code that the
         // whole wave executes whenever any lane needs it)
         unsigned long long needy = __ballot(phase == kNeedPixel);
-        if (__popcll(needy) < kFetchBatch && __ballot(phase == kStart || phase == kTrial || phase == kOuter) != 0)
-            needy = 0;
+        const unsigned long long retry = __ballot(phase == kRetry);
+        const bool serve = __popcll(needy | retry) >= kFetchBatch ||
+                           __ballot(phase == kStart || phase == kTrial || phase == kOuter) == 0;
+        if (!serve) needy = 0;
         long long item = -1;
         while (needy) {
             if (q.next >= q.end) {
@@ -682,10 +685,13 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             q.next += __popcll(took);
             needy &= ~took;
         }
-        if (item >= 0) {
-            pix = dd->list ? dd->list[item] : item;
+        if (item >= 0 || (serve && phase == kRetry)) {
+            // a new pixel starts from the warm-start prediction when there is
+            // one; a retried pixel from the default start
+            if (phase != kRetry) pix = dd->list ? dd->list[item] : item;
+            warm_try = !REFINE && dd->warm != nullptr && phase != kRetry;
             double x0[N], C0[NC], tg0[3], ta0[L];
-            load_pixel<L, REFINE>(*dd, lay, pix, tg0, ta0, bg, x0);
+            load_pixel<L, REFINE>(*dd, lay, pix, tg0, ta0, bg, x0, warm_try);
 #pragma unroll
             for (int c = 0; c < 3; ++c) pc.tgt[c] = tg0[c];
 #pragma unroll
@@ -711,7 +717,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         }
 
         // ---- a drained wave moves to its pending chunk, or ends ----
-        if (__ballot(phase == kStart || phase == kTrial || phase == kOuter) == 0) {
+        if (__ballot(phase == kStart || phase == kTrial || phase == kOuter || phase == kRetry) == 0) {
             if (!q.pending) {
                 q.img = -1;
                 return;
@@ -903,14 +909,14 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             const bool unchanged = __builtin_sqrt(dx2) < sp.eps;
             const bool satisfied = ngn < sp.eps;
             cn_outer = ngn;
-            if ((unchanged && satisfied) || ocount > sp.max_count) {
-                // warm start: a pixel whose solve stopped at the outer-iteration
-                // cap without meeting the plain solve's tolerances (residual
-                // |C| < eps and step < eps) goes to the fallback launch
-                if (!REFINE && dd->fallback && !(unchanged && satisfied)) {
-                    const unsigned long long k = atomicAdd(dd->fallback_count, 1ull);
-                    dd->fallback[k] = pix;
-                }
+            const bool converged = unchanged && satisfied;
+            if (!REFINE && warm_try && !converged && ocount > sp.warm_max_count) {
+                // a warm-started attempt that has not met the plain solve's
+                // tolerances (|C| < eps and step < eps) by its outer cap: the
+                // pixel is solved again from the default start
+                if (dd->fallback_count) atomicAdd(dd->fallback_count, 1ull);
+                phase = kRetry;
+            } else if (converged || ocount > sp.max_count) {
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
                     double* px = dd->out + 4 * ((long long)l * dd->n + pix);
