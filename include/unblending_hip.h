@@ -46,8 +46,10 @@ typedef struct unblending_hip_params {
                                  outer iterations is solved again from the default initialisation */
     const void* mlp;          /* device buffer of a trained model (unblending_hip_mlp_*) */
     int32_t chunk_size;       /* pixels a wave takes from the solve queue at once; 0: default */
-    int32_t filter_block;     /* guided filter workgroup width (256, 512, 1024); 0: chosen by radius */
-    int32_t filter_rows;      /* guided filter rows per workgroup; 0: chosen by image size */
+    int32_t filter_block;     /* guided filter strip width: radius <= 96 one-wave strips of 128 or 256
+                                 columns, larger radii workgroups of 256, 512 or 1024 threads;
+                                 0: chosen by radius */
+    int32_t filter_rows;      /* guided filter rows per strip; 0: chosen by image size */
     int32_t warm_outer_cap;   /* see warm_start; 0: 8 */
 } unblending_hip_params;
 

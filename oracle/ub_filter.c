@@ -139,15 +139,22 @@ void orc_guided_filter(const double* p, const double* rgb, size_t stride, int w,
 /* ---- exact-sum guided filter (the device filter's definition) ----
  *
  * Every box mean is taken over values quantised to a grid 2^-k:
- *   Q(v) = rint(v 2^k) as a 64-bit integer,  mean = ldexp((double)sum Q, -k) * w,
- * w = 1 / (2r + 1)^2, with k chosen per quantity from a bound |v| < 2^E so that
- * every window sum is below 2^62.  Integer sums are exact, so the statistics do
- * not depend on the order of summation (the device's tiling); the per-pixel
- * arithmetic after them is the reference's (image_processing.cpp:226-270).
+ *   Q(v) = rint(v 2^k),  mean = ldexp(sum Q, -k) * w,  w = 1 / (2r + 1)^2,
+ * with k chosen per quantity from a bound |v| <= 2^E so that every partial sum
+ * the device forms stays within 2^53: the device sums the integers Q in FP64,
+ * where such sums are exact, in whatever order its tiling gives; here they are
+ * summed in 64-bit integers.  Either way the statistics do not depend on the
+ * order of summation.  The device's longest partial sums are prefixes over
+ * P(r) columns of (2r + 1)-row window sums, P(r) = 256 for r <= 96 and 1024
+ * beyond, so k = 53 - E - ceil(log2(P(r) (2r + 1))).  The per-pixel
+ * arithmetic after the sums is the reference's (image_processing.cpp:226-270).
  * Bounds: guidance statistics (I, I I) from max|I|; a channel's (p, I p) from
- * max|I| and max|p|; the coefficients a and b from the closed-form bounds
- * |a_c| < 7 max|I| max|p| / eps and |b| < max|p| + 21 max|I|^2 max|p| / eps
- * (|(Sigma + eps I)^-1| <= 2 / eps, |cov| <= 2 sqrt(3) max|I| max|p|).
+ * max|I| and max|p|; the coefficients from |a_c| <= max|p| / sqrt(eps) and
+ * |b| <= max|p| + s3 max|I| max|p| / sqrt(eps), s3 = sqrt(3) rounded up
+ * (twice the bounds |a|_2 <= sd(p) / (2 sqrt(eps)), which hold because the
+ * statistics are, up to the grid step, covariances of one data set, and
+ * sd(p) <= max|p|).  Quantised values are clamped to the bound, so the sums
+ * stay exact whatever the data.
  * r = 0 keeps the reference's identity box (image_processing.hpp:218). */
 
 int orc_qexp(double x)
@@ -159,13 +166,26 @@ int orc_qexp(double x)
     return ex == 0 ? -1022 : ex - 1022; /* smallest e with |x| < 2^e (zero: -1022) */
 }
 
+int orc_qprefix(int r)
+{
+    return r <= 96 ? 256 : 1024;
+}
+
 int orc_qscale(int e, int r)
 {
-    const long long n = (long long)(2 * r + 1) * (2 * r + 1);
+    const long long n = (long long)orc_qprefix(r) * (2 * r + 1);
     int c = 0;
     while ((1LL << c) < n) ++c;
-    int k = 62 - e - c;
+    int k = 53 - e - c;
     return k > 1000 ? 1000 : k;
+}
+
+/* the coefficient bounds of a channel: |a_c| <= ba, |b| <= bb */
+void orc_coeff_bounds(double imax, double pmax, double eps, double* ba, double* bb)
+{
+    const double isr = 1.0 / sqrt(eps);
+    *ba = pmax * isr;
+    *bb = pmax + (ORC_SQRT3_UP * imax) * *ba;
 }
 
 static int64_t quantise(double v, int k, int e)
@@ -254,8 +274,8 @@ void orc_guided_filter_exact(const double* p, const double* rgb, size_t stride, 
     const int eI = orc_qexp(imax), eP = orc_qexp(pmax);
     const int eG = eI > 2 * eI ? eI : 2 * eI; /* I and I I */
     const int eC = eP > eI + eP ? eP : eI + eP; /* p and I p */
-    const double abound = 7.0 * imax * pmax / eps;
-    const double bbound = pmax + 21.0 * imax * imax * pmax / eps;
+    double abound, bbound;
+    orc_coeff_bounds(imax, pmax, eps, &abound, &bbound);
     const int eA = orc_qexp(abound), eB = orc_qexp(bbound);
 
     for (int c = 0; c < 3; ++c) qbox(I[c], w, h, r, eG, mI[c]);

@@ -156,7 +156,11 @@ void orc_guided_filter(const double* p, const double* rgb, size_t stride, int w,
 void orc_guided_filter_exact(const double* p, const double* rgb, size_t stride, int w, int h, int r, double eps,
                              double imax, double pmax, double* q);
 int orc_qexp(double x);
+int orc_qprefix(int r);
 int orc_qscale(int e, int r);
+/* sqrt(3) rounded up: the coefficient bound's constant */
+#define ORC_SQRT3_UP 1.7320508075688774
+void orc_coeff_bounds(double imax, double pmax, double eps, double* ba, double* bb);
 double orc_max_abs(const double* v, size_t n, size_t step);
 int orc_default_radius(int w, int h);
 

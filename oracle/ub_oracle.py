@@ -128,6 +128,11 @@ def lib():
         L.orc_qexp.argtypes = [ctypes.c_double]
         L.orc_qscale.restype = ctypes.c_int
         L.orc_qscale.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_qprefix.restype = ctypes.c_int
+        L.orc_qprefix.argtypes = [ctypes.c_int]
+        L.orc_coeff_bounds.restype = None
+        L.orc_coeff_bounds.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.orc_recomposite.argtypes = [dp, ctypes.c_int, ctypes.c_int, lp, ctypes.c_int, dp]
         _lib = L
     return _lib

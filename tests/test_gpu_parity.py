@@ -90,7 +90,7 @@ def test_guided_filter_matches_oracle(oracle, w, h, r, eps):
         assert np.array_equal(qn, oracle.guided_filter_exact(p, rgb, r, eps))
 
 
-@pytest.mark.parametrize("block,rows", [(256, 7), (512, 0), (1024, 33), (256, 200)])
+@pytest.mark.parametrize("block,rows", [(256, 7), (128, 5), (512, 0), (1024, 33), (256, 200), (128, 0)])
 def test_guided_filter_launch_shapes(oracle, block, rows):
     """Every launch shape the autotuner may pick computes the same filter."""
     import torch

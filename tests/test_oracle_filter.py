@@ -96,9 +96,9 @@ def test_refine_targets_source_over_keeps_alphas(oracle):
         q = np.clip(oracle.guided_filter_exact(np.ascontiguousarray(first[l, ..., 3]), I, 2, 1e-4), 0, 1)
         assert np.abs(t[l] - q).max() == 0
     assert bg is None
-    # the FP64-sum filters give the same targets to rounding
+    # the FP64-sum filters give the same targets to within the coefficient grids (2^-35 here)
     ts, _ = oracle.refine_targets(I, ls, first, radius=2, eps=1e-4, filter="sliding")
-    assert np.abs(ts - t).max() < 1e-11
+    assert np.abs(ts - t).max() < 1e-10
 
 
 def test_refine_targets_plus_normalises_and_nonopaque_zeroes(oracle):
@@ -137,9 +137,20 @@ def _qexp(x):
     return int(e)
 
 
+def _qprefix(r):
+    """longest run of window sums the device adds up for radius r"""
+    return 256 if r <= 96 else 1024
+
+
 def _qscale(e, r):
-    n = (2 * r + 1) ** 2
-    return min(62 - e - int(np.ceil(np.log2(n))), 1000)
+    n = _qprefix(r) * (2 * r + 1)
+    return min(53 - e - int(np.ceil(np.log2(n))), 1000)
+
+
+def _coeff_bounds(imax, pmax, eps):
+    """|a_c| <= pmax / sqrt(eps), |b| <= pmax + s3 imax pmax / sqrt(eps)"""
+    ba = pmax * (1.0 / np.sqrt(eps))
+    return ba, pmax + (1.7320508075688774 * imax) * ba
 
 
 def _qbox_brute(v, r, e):
@@ -158,7 +169,7 @@ def _qbox_brute(v, r, e):
             for dy in range(-r, r + 1):
                 for dx in range(-r, r + 1):
                     s += q[min(max(y + dy, 0), h - 1)][min(max(x + dx, 0), w - 1)]
-            assert abs(s) < 2 ** 62
+            assert abs(s) <= 2 ** 53
             out[y, x] = np.ldexp(float(s), -k) * wgt
     return out
 
@@ -168,8 +179,8 @@ def _exact_filter_numpy(p, I, r, eps):
     imax, pmax = float(np.abs(I).max()), float(np.abs(p).max())
     eI, eP = _qexp(imax), _qexp(pmax)
     eG, eC = max(eI, 2 * eI), max(eP, eI + eP)
-    eA = _qexp(7.0 * imax * pmax / eps)
-    eB = _qexp(pmax + 21.0 * imax * imax * pmax / eps)
+    ba, bb = _coeff_bounds(imax, pmax, eps)
+    eA, eB = _qexp(ba), _qexp(bb)
     Ic = [I[..., c] for c in range(3)]
     mI = [_qbox_brute(Ic[c], r, eG) for c in range(3)]
     mp = _qbox_brute(p, r, eC)
@@ -199,9 +210,21 @@ def _exact_filter_numpy(p, I, r, eps):
 def test_quantisation_exponents(oracle):
     for x, e in [(1.0, 1), (0.99, 0), (0.5, 0), (0.4999, -1), (2.0, 2), (3.0, 2), (0.0, -1022), (-1.5, 1)]:
         assert oracle.lib().orc_qexp(x) == e == _qexp(x), x
-    assert oracle.lib().orc_qscale(1, 1) == 62 - 1 - 4  # 9 = (2r+1)^2 <= 2^4
-    assert oracle.lib().orc_qscale(0, 32) == 62 - 13  # 4225 <= 2^13
+    assert oracle.lib().orc_qscale(1, 1) == 53 - 1 - 10  # 256 * 3 = 768 <= 2^10
+    assert oracle.lib().orc_qscale(0, 32) == 53 - 15  # 256 * 65 = 16640 <= 2^15
+    assert oracle.lib().orc_qscale(0, 96) == 53 - 16  # 256 * 193 = 49408 <= 2^16
+    assert oracle.lib().orc_qscale(0, 97) == 53 - 18  # 1024 * 195 = 199680 <= 2^18
     assert oracle.lib().orc_qscale(-1022, 3) == 1000
+    for r in (1, 8, 32, 96, 97, 480):
+        assert oracle.lib().orc_qprefix(r) == _qprefix(r)
+        # every partial sum of the device (a prefix of qprefix(r) window sums,
+        # each of 2r+1 values of magnitude <= 2^(e+k)) stays within 2^53
+        k = oracle.lib().orc_qscale(3, r)
+        assert _qprefix(r) * (2 * r + 1) * 2.0 ** (3 + k) <= 2.0 ** 53
+    import ctypes
+    ba, bb = ctypes.c_double(), ctypes.c_double()
+    oracle.lib().orc_coeff_bounds(0.9, 0.7, 1e-3, ctypes.byref(ba), ctypes.byref(bb))
+    assert (ba.value, bb.value) == _coeff_bounds(0.9, 0.7, 1e-3)
 
 
 @pytest.mark.parametrize("w,h,r,eps", [(7, 5, 1, 1e-4), (6, 4, 2, 1e-3), (5, 5, 3, 1e-4), (4, 3, 8, 1e-4)])
@@ -225,13 +248,43 @@ def test_exact_filter_is_the_guided_filter_to_rounding(oracle, w, h, r, eps):
     assert np.abs(qe - qb).max() < 1e-9
 
 
+@pytest.mark.parametrize("eps", [1e-4, 1e-3, 1e-2])
+def test_coefficient_bound_holds_with_margin(eps):
+    """The grids of a and b assume |a|_2 <= sd(p) / (2 sqrt(eps)) (the
+    statistics are covariances of one data set; the bound uses twice that).
+    Checked on the textbook filter's coefficients for adversarial guides:
+    nearly flat, nearly collinear channels, and p a function of the guide."""
+    rng = np.random.default_rng(int(eps * 1e5))
+    h, w, r = 24, 20, 3
+    base = rng.uniform(0, 1, (h, w))
+    cases = [rng.uniform(0, 1, (h, w, 3)),
+             np.stack([base, base + 1e-3 * rng.uniform(0, 1, (h, w)), 0.5 * base], -1),
+             0.5 + 1e-4 * rng.uniform(0, 1, (h, w, 3))]
+    for I in cases:
+        for p in (rng.uniform(0, 1, (h, w)), I[..., 0], (I[..., 1] > 0.5).astype(float)):
+            mI = np.stack([box_ref(I[..., c], r) for c in range(3)], -1)
+            mp = box_ref(p, r)
+            cov = np.stack([box_ref(I[..., c] * p, r) for c in range(3)], -1) - mI * mp[..., None]
+            S = np.empty((h, w, 3, 3))
+            for u in range(3):
+                for v in range(3):
+                    S[..., u, v] = box_ref(I[..., u] * I[..., v], r) - mI[..., u] * mI[..., v]
+            a = np.linalg.solve(S + eps * np.eye(3), cov[..., None])[..., 0]
+            sdp = np.sqrt(np.maximum(box_ref(p * p, r) - mp * mp, 0.0))
+            assert np.all(np.linalg.norm(a, axis=-1) <= sdp / (2 * np.sqrt(eps)) * (1 + 1e-6) + 1e-9)
+            ba, bb = _coeff_bounds(float(np.abs(I).max()), float(np.abs(p).max()), eps)
+            b = mp - np.sum(a * mI, -1)
+            assert np.abs(a).max() <= 0.5 * ba * (1 + 1e-6) and np.abs(b).max() <= bb
+
+
 def test_exact_filter_invariants(oracle):
     rng = np.random.default_rng(3)
     I = rng.uniform(0, 1, (11, 13, 3))
     p = rng.uniform(0, 1, (11, 13))
     assert np.array_equal(oracle.guided_filter_exact(p, I, 0, 1e-4), p)  # r = 0: identity
     c = np.full((11, 13), 0.37)
-    assert np.abs(oracle.guided_filter_exact(c, I, 3, 1e-4) - 0.37).max() < 1e-12
+    # within half a step of b's grid: |b| <= 64.4 < 2^7, k = 53 - 7 - 11, step 2^-35
+    assert np.abs(oracle.guided_filter_exact(c, I, 3, 1e-4) - 0.37).max() <= 2.0 ** -36
     # bounds passed explicitly equal to the inputs' give the same result; larger bounds only a coarser grid
     q0 = oracle.guided_filter_exact(p, I, 3, 1e-4)
     q1 = oracle.guided_filter_exact(p, I, 3, 1e-4, imax=float(np.abs(I).max()), pmax=float(np.abs(p).max()))

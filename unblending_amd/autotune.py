@@ -2,9 +2,10 @@
 
 Picks, by timing candidates on the device, the shapes the kernels leave
 open:
-  * the guided filter's workgroup width (256, 512 or 1024 threads; the middle
-    width - 2r columns of a workgroup are its output columns) and rows per
-    workgroup, for an image size, radius and channel count;
+  * the guided filter's strip width and rows per strip, for an image size,
+    radius and channel count: radii up to 96 run one-wave strips of 128 or
+    256 columns, larger radii workgroup strips of 256, 512 or 1024 threads
+    (the middle width - 2r columns of a strip are its output columns);
   * the persistent solve's chunk size (pixels a wave takes from the device
     queue at once), on a sample of the image's pixels.
 Changing a shape does not change what is computed: every candidate runs the
@@ -21,6 +22,8 @@ import numpy as np
 
 from . import hip_params
 
+FILTER_WAVE_MAX_RADIUS = 96
+FILTER_WAVE_BLOCKS = (128, 256)
 FILTER_BLOCKS = (256, 512, 1024)
 FILTER_ROWS = (0, 32, 64, 128, 256, 512)
 CHUNKS = (32, 64, 128, 256, 1024)
@@ -80,6 +83,13 @@ def _time_ms(fn, repeats):
 
 def filter_candidates(radius):
     out = []
+    if 0 < radius <= FILTER_WAVE_MAX_RADIUS:
+        for block in FILTER_WAVE_BLOCKS:
+            if block - 2 * radius < 64:
+                continue
+            for rows in FILTER_ROWS:
+                out.append((block, rows))
+        return out
     for block in FILTER_BLOCKS:
         if block - 2 * radius < 64:
             continue

@@ -426,6 +426,16 @@ UNBLENDING_API int32_t unblending_hip_refine_batch_device(const unblending_hip_i
     return batch_call(images, count, options, params, true, workspace, workspace_size, stream, stats);
 }
 
+// Filter launch shapes: 0 (by radius); radii up to 96 run one-wave strips (128
+// or 256 columns; other widths mean 256), larger radii workgroup strips of
+// 256, 512 or 1024 threads, wider than 2r.
+static bool filter_block_ok(int32_t block, int32_t radius)
+{
+    if (block == 0) return true;
+    if (block != 128 && block != 256 && block != 512 && block != 1024) return false;
+    return radius <= ubhip::kWaveStripMaxRadius || (block != 128 && block - 2 * radius >= 1);
+}
+
 UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_t height, int32_t channels)
 {
     if (width <= 0 || height <= 0 || channels <= 0) return 0;
@@ -442,8 +452,7 @@ UNBLENDING_API int32_t unblending_hip_guided_filter_device(const double* rgb, in
     g_last_error.clear();
     try {
         if (!rgb || !p || !q || channels <= 0 || width <= 0 || height <= 0 || radius < 0 || !(epsilon > 0.0) ||
-            (block != 0 && block != 256 && block != 512 && block != 1024) || rows < 0 ||
-            (block != 0 && block - 2 * radius < 1))
+            !filter_block_ok(block, radius) || rows < 0)
             return UNBLENDING_INVALID_ARGUMENT;
         if (!workspace || workspace_size < filter_workspace_bytes(size_t(width) * size_t(height), channels))
             return UNBLENDING_BUFFER_TOO_SMALL;

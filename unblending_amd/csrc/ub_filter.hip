@@ -2,30 +2,45 @@
 //
 // Semantics: unblending/src/image_processing.cpp:214-273 with the clamp-edge
 // box of image_processing.cpp:120-151 / image_processing.hpp:215-222, every
-// box mean taken over values quantised to a grid 2^-k and summed in 64-bit
-// integers:
-//   Q(v) = rint(v 2^k),  mean = ldexp((double) sum Q, -k) * (1 / (2r + 1)^2),
-// with k per quantity from a bound |v| < 2^E such that every window sum stays
-// below 2^62 (guidance statistics I, I I from max|I|; a channel's p, I p from
-// max|I| and max|p|; the coefficients from the closed-form bounds
-// |a| < 7 max|I| max|p| / eps, |b| < max|p| + 21 max|I|^2 max|p| / eps).
-// Integer sums are exact, so the statistics do not depend on the order of
-// summation: the tiling below (and any launch shape the autotuner picks)
+// box mean taken over values quantised to a grid 2^-k:
+//   Q(v) = rint(v 2^k),  mean = (sum Q) 2^-k (1 / (2r + 1)^2).
+// The integers Q are summed in FP64, where sums stay exact while they are
+// within 2^53: k per quantity is chosen from a bound |v| <= 2^e so that the
+// longest partial sum the kernels form -- a prefix over P(r) columns of
+// (2r + 1)-row window sums, P(r) = 256 for r <= 96 (one-wave strips of at most
+// 256 columns) and 1024 beyond (workgroup strips of at most 1024 columns) --
+// stays within 2^53: k = 53 - e - ceil(log2(P(r) (2r + 1))).  Bounds:
+// guidance statistics (I, I I) from max|I|; a channel's (p, I p) from max|I|
+// and max|p|; the coefficients from |a_c| <= max|p| / sqrt(eps) and
+// |b| <= max|p| + s3 max|I| max|p| / sqrt(eps) (twice the bound
+// |a|_2 <= sd(p) / (2 sqrt(eps)) that holds when the statistics are
+// covariances of one data set; coefficients are clamped to the bound, so the
+// sums stay exact whatever the data).  Exact sums do not depend on the order
+// of summation: the tiling below, and any launch shape the autotuner picks,
 // gives the host oracle's result bit for bit.  The per-pixel arithmetic after
-// the sums is the reference's, in its order.
+// the sums is the reference's, in its order.  No sum is ever -0.0 (the
+// accumulators start at +0.0 and only add), as in the oracle's integer sums.
 //
 // Kernels:
 //   bounds   max|I| and max|p| of every channel (one pass over the inputs)
-//   pass A   per channel group: box statistics (9 guidance + 4 per channel),
-//            a = (Sigma + eps I)^-1 cov_Ip and b = mean_p - a . mean_I
-//   pass B   per channel group: box of (a_r, a_g, a_b, b) and
-//            q = box(b) + box(a_r) I_r + box(a_g) I_g + box(a_b) I_b
-// A strip workgroup of E threads owns E consecutive (clamped) columns, of
-// which the middle TW = E - 2r are its output columns, and walks RH rows.
-// Each thread keeps its column's vertical window sums in registers (add the
-// entering row, subtract the leaving row, whose inputs are fetched one row
-// ahead); a row's horizontal window sums are differences of a workgroup-wide
-// inclusive prefix sum (wave scans, then one LDS exchange for all statistics).
+//   pass A   box statistics (9 guidance + 4 per channel) and the per-pixel
+//            coefficients a = (Sigma + eps I)^-1 cov_Ip, b = mean_p - a . mean_I
+//            (r <= 96: 1 to 3 channels a wave, the guidance statistics
+//            recomputed by each wave: reading stored guidance terms at every
+//            output pixel of every channel costs more than the recomputation)
+//   pass B   box of (a_r, a_g, a_b, b) and q = box(b) + sum_c box(a_c) I_c
+// r <= 96: one-wave strips.  A wave owns 64 K consecutive (clamped) columns,
+// K per lane, for one channel, and walks RH rows: vertical window sums are
+// running sums in registers (the entering and the leaving rows are fetched a
+// row ahead); a row's horizontal window sums are differences of the strip's
+// prefix sums (a lane's own K columns, then a DPP scan of the lane totals),
+// the left end fetched from the lane 2r + 1 columns back by ds_bpermute.  No
+// LDS and no barriers.  The waves of one tile (its strip and rows, every
+// channel group) run back to back on one XCD, so that they share its L2 for
+// the guidance, and each XCD takes a contiguous run of tiles (neighbouring
+// strips).
+// r > 96: workgroup strips of E threads, one column each, the horizontal
+// prefix combined across the workgroup's waves through LDS.
 // r = 0 is the identity (the reference's box returns its input, so q = p).
 #include "ub_kernels.hpp"
 
@@ -42,29 +57,30 @@ __host__ __device__ __forceinline__ int qexp(double x)
     return ex == 0 ? -1022 : ex - 1022;
 }
 
-// the grid exponent k of a quantity bounded by 2^e in windows of radius r; orc_qscale
+// the grid exponent k of a quantity bounded by 2^e for radius r; orc_qscale
 __device__ __forceinline__ int qscale(int e, int r)
 {
-    const long long n = (long long)(2 * r + 1) * (2 * r + 1);
+    const long long n = (long long)(r <= kWaveStripMaxRadius ? 256 : 1024) * (2 * r + 1);
     int c = 0;
     while ((1ll << c) < n) ++c;
-    const int k = 62 - e - c;
+    const int k = 53 - e - c;
     return k > 1000 ? 1000 : k;
 }
 
-// a quantiser: grid 2^-k, values bounded by 2^e (clamped there, which bounded
-// values never reach)
+// a quantiser: grid 2^-k, values bounded by 2^e
 struct Quant {
     double scale;  // 2^k
     double lim;    // 2^(e + k)
     double back;   // 2^-k
-    __device__ __forceinline__ long long q(double v) const
+    // values known to lie within the bound
+    __device__ __forceinline__ double q(double v) const { return __builtin_rint(v * scale); }
+    // values clamped to the bound (which bounded values never reach)
+    __device__ __forceinline__ double qc(double v) const
     {
-        double t = __builtin_rint(v * scale);
-        t = t > lim ? lim : (t < -lim ? -lim : t);
-        return (long long)t;
+        const double t = __builtin_rint(v * scale);
+        return __builtin_fmin(__builtin_fmax(t, -lim), lim);
     }
-    __device__ __forceinline__ double mean(long long s, double weight) const { return ((double)s * back) * weight; }
+    __device__ __forceinline__ double mean(double s, double weight) const { return (s * back) * weight; }
 };
 
 __device__ __forceinline__ Quant make_quant(int e, int r)
@@ -82,31 +98,144 @@ __device__ __forceinline__ double bound_of(const FilterArgs& a, int slot)
     return __builtin_bit_cast(double, (long long)a.bounds[slot]);
 }
 
-// ---- bounds: max|I| (slot 0) and max|p_c| (slot 1 + c), all channels in one pass ----
+constexpr double kSqrt3Up = 1.7320508075688774;  // sqrt(3) rounded up; the oracle's ORC_SQRT3_UP
+
+// grids of the guidance statistics (I and I I) and of one channel's (p and I p)
+__device__ __forceinline__ void stat_quants(const FilterArgs& a, int ch, Quant& qg, Quant& qc)
+{
+    const int eI = qexp(bound_of(a, 0));
+    const int eP = qexp(bound_of(a, 1 + ch));
+    qg = make_quant(eI > 2 * eI ? eI : 2 * eI, a.r);
+    qc = make_quant(eP > eI + eP ? eP : eI + eP, a.r);
+}
+
+// grids of one channel's coefficients: |a_c| <= pmax / sqrt(eps), |b| <= pmax + s3 imax pmax / sqrt(eps)
+__device__ __forceinline__ void coeff_quants(const FilterArgs& a, int ch, Quant& qa, Quant& qb)
+{
+    const double imax = bound_of(a, 0), pmax = bound_of(a, 1 + ch);
+    const double ba = pmax * a.isr;
+    const double bb = pmax + (kSqrt3Up * imax) * ba;
+    qa = make_quant(qexp(ba), a.r);
+    qb = make_quant(qexp(bb), a.r);
+}
+
+// The per-pixel guidance terms of image_processing.cpp:247-258 from the
+// window sums G[0..2] I, G[3..8] I I (00 01 02 11 12 22): the entries of
+// (Sigma + eps I)^-1 as cofactor * (1 / det) products (00 01 02 11 12 22),
+// then mean_I (3).
+__device__ __forceinline__ void guidance_terms(const double* G, const Quant& qg, double weight, double eps,
+                                               double gi[9])
+{
+    const double mI0 = qg.mean(G[0], weight), mI1 = qg.mean(G[1], weight), mI2 = qg.mean(G[2], weight);
+    const double m00 = (qg.mean(G[3], weight) - mI0 * mI0) + eps;
+    const double m01 = qg.mean(G[4], weight) - mI0 * mI1;
+    const double m02 = qg.mean(G[5], weight) - mI0 * mI2;
+    const double m11 = (qg.mean(G[6], weight) - mI1 * mI1) + eps;
+    const double m12 = qg.mean(G[7], weight) - mI1 * mI2;
+    const double m22 = (qg.mean(G[8], weight) - mI2 * mI2) + eps;
+    const double c00 = m11 * m22 - m12 * m12;
+    const double c01 = m02 * m12 - m01 * m22;
+    const double c02 = m01 * m12 - m02 * m11;
+    const double c11 = m00 * m22 - m02 * m02;
+    const double c12 = m01 * m02 - m00 * m12;
+    const double c22 = m00 * m11 - m01 * m01;
+    const double det = m00 * c00 + m01 * c01 + m02 * c02;
+    const double id = 1.0 / det;
+    gi[0] = c00 * id;
+    gi[1] = c01 * id;
+    gi[2] = c02 * id;
+    gi[3] = c11 * id;
+    gi[4] = c12 * id;
+    gi[5] = c22 * id;
+    gi[6] = mI0;
+    gi[7] = mI1;
+    gi[8] = mI2;
+}
+
+// a = (Sigma + eps I)^-1 cov_Ip and b = mean_p - a . mean_I
+// (image_processing.cpp:258-265) from the guidance terms and a channel's
+// window sums P[0] p, P[1..3] I p
+__device__ __forceinline__ void channel_coefficients(const double* gi, const double* P, const Quant& qc,
+                                                     double weight, double out[4])
+{
+    const double mp = qc.mean(P[0], weight);
+    const double v0 = qc.mean(P[1], weight) - gi[6] * mp;
+    const double v1 = qc.mean(P[2], weight) - gi[7] * mp;
+    const double v2 = qc.mean(P[3], weight) - gi[8] * mp;
+    const double ar = gi[0] * v0 + gi[1] * v1 + gi[2] * v2;
+    const double ag = gi[1] * v0 + gi[3] * v1 + gi[4] * v2;
+    const double ab = gi[2] * v0 + gi[4] * v1 + gi[5] * v2;
+    out[0] = ar;
+    out[1] = ag;
+    out[2] = ab;
+    out[3] = ((mp - ar * gi[6]) - ag * gi[7]) - ab * gi[8];
+}
+
+// the 9 guidance statistics of one pixel, quantised
+__device__ __forceinline__ void guidance_stats(const Quant& qg, double I0, double I1, double I2, double* s)
+{
+    s[0] = qg.q(I0);
+    s[1] = qg.q(I1);
+    s[2] = qg.q(I2);
+    s[3] = qg.q(I0 * I0);
+    s[4] = qg.q(I0 * I1);
+    s[5] = qg.q(I0 * I2);
+    s[6] = qg.q(I1 * I1);
+    s[7] = qg.q(I1 * I2);
+    s[8] = qg.q(I2 * I2);
+}
+
+// a channel's 4 statistics of one pixel, quantised
+__device__ __forceinline__ void channel_stats(const Quant& qc, double I0, double I1, double I2, double p, double* s)
+{
+    s[0] = qc.q(p);
+    s[1] = qc.q(I0 * p);
+    s[2] = qc.q(I1 * p);
+    s[3] = qc.q(I2 * p);
+}
+
+// ---- bounds: max|I| (slot 0) and max|p_c| (slot 1 + c) ----
+// One source after another (the guidance as 3n values, then each channel),
+// contiguous sources read as 16-byte pairs, four in flight per thread.
+__device__ __forceinline__ double absmax2(double m, double2 v)
+{
+    return __builtin_fmax(m, __builtin_fmax(__builtin_fabs(v.x), __builtin_fabs(v.y)));
+}
+
 __global__ void __launch_bounds__(256) bounds_kernel(BoundsArgs a)
 {
     const long long n = (long long)a.w * a.h;
-    double mi = 0.0, mp[kMaxChannels];
-#pragma unroll
-    for (int g = 0; g < kMaxChannels; ++g) mp[g] = 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) mi = __builtin_fmax(mi, __builtin_fabs(a.rgb[3 * i + c]));
-#pragma unroll
-        for (int g = 0; g < kMaxChannels; ++g)
-            if (g < a.channels) mp[g] = __builtin_fmax(mp[g], __builtin_fabs(a.in[g].base[i * a.in[g].stride]));
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        mi = __builtin_fmax(mi, __shfl_xor(mi, off));
-#pragma unroll
-        for (int g = 0; g < kMaxChannels; ++g) mp[g] = __builtin_fmax(mp[g], __shfl_xor(mp[g], off));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        // non-negative doubles order as their bit patterns
-        atomicMax(&a.bounds[0], (unsigned long long)__builtin_bit_cast(long long, mi));
-#pragma unroll
-        for (int g = 0; g < kMaxChannels; ++g)
-            if (g < a.channels) atomicMax(&a.bounds[1 + g], (unsigned long long)__builtin_bit_cast(long long, mp[g]));
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nth = (long long)gridDim.x * blockDim.x;
+    for (int src = -1; src < a.channels; ++src) {
+        const double* base = src < 0 ? a.rgb : a.in[src].base;
+        const long long count = src < 0 ? 3 * n : n;
+        const int stride = src < 0 ? 1 : a.in[src].stride;
+        double m = 0.0;
+        if (stride == 1 && ((unsigned long long)base & 15) == 0) {
+            const double2* p2 = reinterpret_cast<const double2*>(base);
+            const long long pairs = count >> 1;
+            long long i = tid;
+            for (; i + 3 * nth < pairs; i += 4 * nth) {
+                const double2 v0 = p2[i], v1 = p2[i + nth], v2 = p2[i + 2 * nth], v3 = p2[i + 3 * nth];
+                m = absmax2(absmax2(m, v0), v1);
+                m = absmax2(absmax2(m, v2), v3);
+            }
+            for (; i < pairs; i += nth) m = absmax2(m, p2[i]);
+            if ((count & 1) && tid == 0) m = __builtin_fmax(m, __builtin_fabs(base[count - 1]));
+        } else {
+            for (long long i = tid; i < count; i += nth) m = __builtin_fmax(m, __builtin_fabs(base[i * stride]));
+        }
+        for (int off = 32; off > 0; off >>= 1) m = __builtin_fmax(m, __shfl_xor(m, off));
+        // one atomic per workgroup and source; non-negative doubles order as their bit patterns
+        __shared__ double part[4];
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            m = __builtin_fmax(__builtin_fmax(part[0], part[1]), __builtin_fmax(part[2], part[3]));
+            atomicMax(&a.bounds[src + 1], (unsigned long long)__builtin_bit_cast(long long, m));
+        }
+        __syncthreads();
     }
 }
 
@@ -116,15 +245,408 @@ __global__ void copy_kernel(FilterArgs a)
     const long long n = (long long)a.w * a.h;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-#pragma unroll
-    for (int g = 0; g < kMaxGroup; ++g)
-        if (g < a.groups) a.q[g][i] = a.in[g].base[i * a.in[g].stride];
+    for (int g = 0; g < a.channels; ++g) a.q[g][i] = a.in[g].base[i * a.in[g].stride];
 }
 
-// ---- the sources of the two passes ----
+// v of the lane the DPP control selects (+0.0 where it selects none)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dppd(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 
-// Pass A: raw inputs I (3) and p (G); statistics I (3), I I (6), then per
-// channel p, I_r p, I_g p, I_b p.
+// inclusive prefix sum across the 64 lanes of a wave: shifts by 1, 2, 4, 8
+// within rows of 16 lanes, then the row broadcasts of lanes 15 and 31 (GFX9 DPP)
+__device__ __forceinline__ double wave_scan(double v)
+{
+    v += dppd<0x111, 0xf>(v);  // row_shr:1
+    v += dppd<0x112, 0xf>(v);  // row_shr:2
+    v += dppd<0x114, 0xf>(v);  // row_shr:4
+    v += dppd<0x118, 0xf>(v);  // row_shr:8
+    v += dppd<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dppd<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// v of lane `byte_index / 4` (any lane of the wave)
+__device__ __forceinline__ double lane_fetch(double v, int byte_index)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(byte_index, (int)(unsigned)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_bpermute(byte_index, (int)(b >> 32));
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// ===========================================================================
+// One-wave strips (r <= 96)
+// ===========================================================================
+
+// K values of a channel from the columns cols[] (clamped) of a row
+template <int K>
+__device__ __forceinline__ void load_channel(const Channel& ch, long long rowbase, const int* cols, double* v, int step)
+{
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j * step] = ch.base[(rowbase + cols[j]) * ch.stride];
+}
+
+template <int K>
+__device__ __forceinline__ void load_rgb(const double* rgb, long long rowbase, const int* cols, double* v, int step)
+{
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const double* p = rgb + 3 * (rowbase + cols[j]);
+        v[j * step] = p[0];
+        v[j * step + 1] = p[1];
+        v[j * step + 2] = p[2];
+    }
+}
+
+// Pass A, the guidance wave of a tile: raw inputs (I_r, I_g, I_b), the 9
+// guidance statistics.
+struct GuideSrc {
+    static constexpr int NR = 3;
+    static constexpr int NS = 9;
+    const double* rgb;
+    int w;
+    double eps, weight;
+    Quant qg;
+    __device__ __forceinline__ void init(const FilterArgs& a)
+    {
+        rgb = a.rgb;
+        w = a.w;
+        eps = a.eps;
+        weight = 1.0 / (double)((2 * a.r + 1) * (2 * a.r + 1));
+        Quant qc;
+        stat_quants(a, 0, qg, qc);
+    }
+    template <int K>
+    __device__ __forceinline__ void load(long long rowbase, const int* cols, double v[][NR]) const
+    {
+        load_rgb<K>(rgb, rowbase, cols, &v[0][0], NR);
+    }
+    __device__ __forceinline__ void stats(const double* v, double* s) const { guidance_stats(qg, v[0], v[1], v[2], s); }
+};
+
+// Pass A, a channel wave of a tile: raw inputs (I_r, I_g, I_b, p), the
+// statistics p, I_r p, I_g p, I_b p; its coefficients combine them with the
+// guidance terms of the tile's guidance wave.
+struct ChanSrc {
+    static constexpr int NR = 4;
+    static constexpr int NS = 4;
+    const double* rgb;
+    Channel in;
+    double* ab;  // this channel's 4 planes
+    long long n;
+    int w;
+    double weight;
+    Quant qc;
+    __device__ __forceinline__ void init(const FilterArgs& a, int ch)
+    {
+        rgb = a.rgb;
+        in = a.in[ch];
+        n = (long long)a.w * a.h;
+        ab = a.ab + (long long)(4 * ch) * n;
+        w = a.w;
+        weight = 1.0 / (double)((2 * a.r + 1) * (2 * a.r + 1));
+        Quant qg;
+        stat_quants(a, ch, qg, qc);
+    }
+    template <int K>
+    __device__ __forceinline__ void load(long long rowbase, const int* cols, double v[][NR]) const
+    {
+        load_rgb<K>(rgb, rowbase, cols, &v[0][0], NR);
+        load_channel<K>(in, rowbase, cols, &v[0][3], NR);
+    }
+    __device__ __forceinline__ void stats(const double* v, double* s) const
+    {
+        channel_stats(qc, v[0], v[1], v[2], v[3], s);
+    }
+    __device__ __forceinline__ void out(long long pix, const double* gi, const double* H) const
+    {
+        double c[4];
+        channel_coefficients(gi, H, qc, weight, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ab[(long long)k * n + pix] = c[k];
+    }
+};
+
+// Pass B of one channel: raw inputs and statistics (a_r, a_g, a_b, b),
+// output q = box(b) + box(a_r) I_r + box(a_g) I_g + box(a_b) I_b
+// (image_processing.cpp:267-270, that order).
+struct WaveB {
+    static constexpr int NR = 4;
+    static constexpr int NS = 4;
+    const double* rgb;
+    const double* ab;
+    double* q;
+    long long n;
+    int w;
+    double weight;
+    Quant qa, qb;
+    __device__ __forceinline__ void init(const FilterArgs& a, int ch)
+    {
+        rgb = a.rgb;
+        n = (long long)a.w * a.h;
+        ab = a.ab + (long long)(4 * ch) * n;
+        q = a.q[ch];
+        w = a.w;
+        weight = 1.0 / (double)((2 * a.r + 1) * (2 * a.r + 1));
+        coeff_quants(a, ch, qa, qb);
+    }
+    template <int K>
+    __device__ __forceinline__ void load(long long rowbase, const int* cols, double v[][NR]) const
+    {
+        // in-plane offsets fit 32 bits (n <= (2^31 - 1) / 16 on the device)
+        const int row = (int)rowbase;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double* pk = ab + (long long)k * n;
+#pragma unroll
+            for (int j = 0; j < K; ++j) v[j][k] = pk[row + cols[j]];
+        }
+    }
+    __device__ __forceinline__ void stats(const double* v, double* s) const
+    {
+        s[0] = qa.qc(v[0]);
+        s[1] = qa.qc(v[1]);
+        s[2] = qa.qc(v[2]);
+        s[3] = qb.qc(v[3]);
+    }
+    __device__ __forceinline__ void out(long long pix, const double* H) const
+    {
+        const double I0 = rgb[3 * pix], I1 = rgb[3 * pix + 1], I2 = rgb[3 * pix + 2];
+        double v = qb.mean(H[3], weight);
+        v = v + qa.mean(H[0], weight) * I0;
+        v = v + qa.mean(H[1], weight) * I1;
+        v = v + qa.mean(H[2], weight) * I2;
+        q[pix] = v;
+    }
+};
+
+// One wave's share of a strip: input columns X0 - r + [0, 64 K), K per lane;
+// output columns X0 + [0, TW), TW = 64 K - 2r.  Holds the running vertical
+// window sums of the wave's statistics and the next row's inputs.  RHO =
+// (2r + 1) mod K fixes, per slot j of a lane, which slot of which lane holds
+// the prefix just left of its window.
+template <class Src, int K, int RHO>
+struct StripRows {
+    static constexpr int NS = Src::NS;
+    static constexpr int NR = Src::NR;
+    double V[K][NS];
+    double vin[K][NR], vout[K][NR];
+    int cols[K];  // the lane's input columns, clamped to the image
+    int lane, r, w, h, base;
+
+    __device__ __forceinline__ void setup(int X0, int r_, int w_, int h_)
+    {
+        lane = threadIdx.x & 63;
+        r = r_;
+        w = w_;
+        h = h_;
+#pragma unroll
+        for (int j = 0; j < K; ++j) cols[j] = clampi(X0 - r + lane * K + j, 0, w - 1);
+        // the prefix just left of slot j's window, column lane K + j - (2r + 1):
+        // lane lane - base - (j < RHO), slot (j - RHO) mod K
+        base = (2 * r + 1 - RHO) / K;
+    }
+    // window sums of row y0 (rows y0 - r .. y0 + r)
+    __device__ __forceinline__ void warmup(const Src& src, int y0)
+    {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) V[j][k] = 0.0;
+        for (int dy = -r; dy <= r; ++dy) {
+            double v[K][NR];
+            src.template load<K>((long long)clampi(y0 + dy, 0, h - 1) * w, cols, v);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double s[NS];
+                src.stats(v[j], s);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) V[j][k] += s[k];
+            }
+        }
+    }
+    // to row y (from y - 1 unless first), then fetch the inputs of row y + 1's update
+    __device__ __forceinline__ void advance(const Src& src, int y, bool first, bool more)
+    {
+        if (!first) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double si[NS], so[NS];
+                src.stats(vin[j], si);
+                src.stats(vout[j], so);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) V[j][k] = V[j][k] + (si[k] - so[k]);
+            }
+        }
+        if (more) {
+            src.template load<K>((long long)clampi(y + 1 + r, 0, h - 1) * w, cols, vin);
+            src.template load<K>((long long)clampi(y - r, 0, h - 1) * w, cols, vout);
+        }
+    }
+    // window sums of the row: H = S(c) - S(c - 2r - 1), S the strip's prefix
+    __device__ __forceinline__ void window(double H[K][NS]) const
+    {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            double S[K];
+            S[0] = V[0][k];
+#pragma unroll
+            for (int j = 1; j < K; ++j) S[j] = S[j - 1] + V[j][k];
+            const double T = S[K - 1];
+            const double ex = wave_scan(T) - T;
+#pragma unroll
+            for (int j = 0; j < K; ++j) S[j] = ex + S[j];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const int from = lane - base - (j < RHO ? 1 : 0);
+                const double left = lane_fetch(S[(j - RHO + K) % K], from << 2);
+                H[j][k] = S[j] - (from >= 0 ? left : 0.0);
+            }
+        }
+    }
+};
+
+// Tile of a launch: block b runs on XCD b mod 8; the channel groups of a tile
+// are consecutive blocks of one XCD, and each XCD takes a contiguous run of
+// tiles (neighbouring strips of the same rows).  Returns false past the end.
+struct TilePos {
+    int tile, group, strip, chunk;
+};
+__device__ __forceinline__ bool tile_of_block(int tiles, int strips, int groups, TilePos& t)
+{
+    const int xcd = blockIdx.x & 7;
+    const int local = blockIdx.x >> 3;
+    t.group = local % groups;
+    const int tl = local / groups;
+    const int per_xcd = (tiles + 7) >> 3;
+    t.tile = xcd * per_xcd + tl;
+    if (tl >= per_xcd || t.tile >= tiles) return false;
+    t.strip = t.tile % strips;
+    t.chunk = t.tile / strips;
+    return true;
+}
+
+// Pass A, r <= 96: a workgroup of 1 + S waves per tile and channel group.
+// Wave 0 keeps the guidance statistics and, each row, writes the guidance
+// terms of its output columns to LDS; waves 1..S each keep one channel's
+// statistics and, after the row's barrier, combine them with the guidance
+// terms into the channel's coefficients.  The channel waves load the same
+// guidance rows as wave 0, from the CU's cache.
+template <int K, int RHO>
+__global__ void __launch_bounds__(512) coeff_tile_kernel(FilterArgs a, int TW, int RH, int strips, int tiles, int S)
+{
+    __shared__ double gt[2][9][K][64];
+    const int groups = (a.channels + S - 1) / S;
+    TilePos t;
+    if (!tile_of_block(tiles, strips, groups, t)) return;  // the whole workgroup
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int X0 = t.strip * TW;
+    const int y0 = t.chunk * RH;
+    const int y1 = min(a.h, y0 + RH);
+    const int r = a.r;
+
+    if (wave == 0) {
+        GuideSrc src;
+        src.init(a);
+        StripRows<GuideSrc, K, RHO> st;
+        st.setup(X0, r, a.w, a.h);
+        st.warmup(src, y0);
+        int buf = 0;
+        for (int y = y0; y < y1; ++y) {
+            st.advance(src, y, y == y0, y + 1 < y1);
+            double H[K][9];
+            st.window(H);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double gi[9];
+                guidance_terms(H[j], src.qg, src.weight, src.eps, gi);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) gt[buf][k][j][lane] = gi[k];
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+        return;
+    }
+    const int ch = t.group * S + wave - 1;
+    if (ch >= a.channels) {
+        for (int y = y0; y < y1; ++y) __syncthreads();  // an idle wave keeps the barrier count
+        return;
+    }
+    ChanSrc src;
+    src.init(a, ch);
+    StripRows<ChanSrc, K, RHO> st;
+    st.setup(X0, r, a.w, a.h);
+    st.warmup(src, y0);
+    int buf = 0;
+    for (int y = y0; y < y1; ++y) {
+        st.advance(src, y, y == y0, y + 1 < y1);
+        double H[K][4];
+        st.window(H);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int c = lane * K + j;  // strip column; its window is strip columns c - 2r .. c
+            const int x = X0 + c - 2 * r;
+            if (c >= 2 * r && c - 2 * r < TW && x < a.w) {
+                double gi[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) gi[k] = gt[buf][k][j][lane];
+                src.out((long long)y * a.w + x, gi, H[j]);
+            }
+        }
+        buf ^= 1;
+    }
+}
+
+// Pass B, r <= 96: a workgroup of S waves per tile and channel group, one
+// channel each (the waves read the same guidance at their output pixels,
+// from the CU's cache).
+template <int K, int RHO>
+__global__ void __launch_bounds__(512) output_tile_kernel(FilterArgs a, int TW, int RH, int strips, int tiles, int S)
+{
+    const int groups = (a.channels + S - 1) / S;
+    TilePos t;
+    if (!tile_of_block(tiles, strips, groups, t)) return;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int ch = t.group * S + wave;
+    if (ch >= a.channels) return;
+    const int X0 = t.strip * TW;
+    const int y0 = t.chunk * RH;
+    const int y1 = min(a.h, y0 + RH);
+    const int r = a.r;
+    WaveB src;
+    src.init(a, ch);
+    StripRows<WaveB, K, RHO> st;
+    st.setup(X0, r, a.w, a.h);
+    st.warmup(src, y0);
+    for (int y = y0; y < y1; ++y) {
+        st.advance(src, y, y == y0, y + 1 < y1);
+        double H[K][4];
+        st.window(H);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int c = lane * K + j;
+            const int x = X0 + c - 2 * r;
+            if (c >= 2 * r && c - 2 * r < TW && x < a.w) src.out((long long)y * a.w + x, H[j]);
+        }
+    }
+}
+
+// ===========================================================================
+// Workgroup strips (r > 96)
+// ===========================================================================
+
+// Pass A of a group of G channels (first_channel ...): raw inputs I (3) and
+// p (G); statistics I (3), I I (6), then per channel p, I_r p, I_g p, I_b p.
 template <int G>
 struct PassA {
     static constexpr int NR = 3 + G;
@@ -135,14 +657,8 @@ struct PassA {
 
     __device__ __forceinline__ void init()
     {
-        const double imax = bound_of(a, 0);
-        const int eI = qexp(imax);
-        qg = make_quant(eI > 2 * eI ? eI : 2 * eI, a.r);
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int eP = qexp(bound_of(a, 1 + a.first_channel + g));
-            qc[g] = make_quant(eP > eI + eP ? eP : eI + eP, a.r);
-        }
+        for (int g = 0; g < G; ++g) stat_quants(a, a.first_channel + g, qg, qc[g]);
         weight = 1.0 / (double)((2 * a.r + 1) * (2 * a.r + 1));
     }
     __device__ __forceinline__ void raw(long long pix, double* v) const
@@ -151,65 +667,33 @@ struct PassA {
         v[1] = a.rgb[3 * pix + 1];
         v[2] = a.rgb[3 * pix + 2];
 #pragma unroll
-        for (int g = 0; g < G; ++g) v[3 + g] = a.in[g].base[pix * a.in[g].stride];
-    }
-    __device__ __forceinline__ void stats(const double* v, long long* s) const
-    {
-        const double I0 = v[0], I1 = v[1], I2 = v[2];
-        s[0] = qg.q(I0);
-        s[1] = qg.q(I1);
-        s[2] = qg.q(I2);
-        s[3] = qg.q(I0 * I0);
-        s[4] = qg.q(I0 * I1);
-        s[5] = qg.q(I0 * I2);
-        s[6] = qg.q(I1 * I1);
-        s[7] = qg.q(I1 * I2);
-        s[8] = qg.q(I2 * I2);
-#pragma unroll
         for (int g = 0; g < G; ++g) {
-            const double p = v[3 + g];
-            s[9 + 4 * g] = qc[g].q(p);
-            s[10 + 4 * g] = qc[g].q(I0 * p);
-            s[11 + 4 * g] = qc[g].q(I1 * p);
-            s[12 + 4 * g] = qc[g].q(I2 * p);
+            const Channel& c = a.in[a.first_channel + g];
+            v[3 + g] = c.base[pix * c.stride];
         }
     }
-    __device__ __forceinline__ void out(long long pix, long long n, const long long* H) const
+    __device__ __forceinline__ void stats(const double* v, double* s) const
     {
-        const double mI0 = qg.mean(H[0], weight), mI1 = qg.mean(H[1], weight), mI2 = qg.mean(H[2], weight);
-        const double m00 = (qg.mean(H[3], weight) - mI0 * mI0) + a.eps;
-        const double m01 = qg.mean(H[4], weight) - mI0 * mI1;
-        const double m02 = qg.mean(H[5], weight) - mI0 * mI2;
-        const double m11 = (qg.mean(H[6], weight) - mI1 * mI1) + a.eps;
-        const double m12 = qg.mean(H[7], weight) - mI1 * mI2;
-        const double m22 = (qg.mean(H[8], weight) - mI2 * mI2) + a.eps;
-        const double c00 = m11 * m22 - m12 * m12;
-        const double c01 = m02 * m12 - m01 * m22;
-        const double c02 = m01 * m12 - m02 * m11;
-        const double c11 = m00 * m22 - m02 * m02;
-        const double c12 = m01 * m02 - m00 * m12;
-        const double c22 = m00 * m11 - m01 * m01;
-        const double det = m00 * c00 + m01 * c01 + m02 * c02;
-        const double id = 1.0 / det;
+        guidance_stats(qg, v[0], v[1], v[2], s);
+#pragma unroll
+        for (int g = 0; g < G; ++g) channel_stats(qc[g], v[0], v[1], v[2], v[3 + g], s + 9 + 4 * g);
+    }
+    __device__ __forceinline__ void out(long long pix, long long n, const double* H) const
+    {
+        double gi[9];
+        guidance_terms(H, qg, weight, a.eps, gi);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const double mp = qc[g].mean(H[9 + 4 * g], weight);
-            const double v0 = qc[g].mean(H[10 + 4 * g], weight) - mI0 * mp;
-            const double v1 = qc[g].mean(H[11 + 4 * g], weight) - mI1 * mp;
-            const double v2 = qc[g].mean(H[12 + 4 * g], weight) - mI2 * mp;
-            const double ar = (c00 * id) * v0 + (c01 * id) * v1 + (c02 * id) * v2;
-            const double ag = (c01 * id) * v0 + (c11 * id) * v1 + (c12 * id) * v2;
-            const double ab = (c02 * id) * v0 + (c12 * id) * v1 + (c22 * id) * v2;
-            const double b = ((mp - ar * mI0) - ag * mI1) - ab * mI2;
-            a.ab[(long long)(4 * g + 0) * n + pix] = ar;
-            a.ab[(long long)(4 * g + 1) * n + pix] = ag;
-            a.ab[(long long)(4 * g + 2) * n + pix] = ab;
-            a.ab[(long long)(4 * g + 3) * n + pix] = b;
+            double c[4];
+            channel_coefficients(gi, H + 9 + 4 * g, qc[g], weight, c);
+            double* ab = a.ab + (long long)(4 * (a.first_channel + g)) * n;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ab[(long long)k * n + pix] = c[k];
         }
     }
 };
 
-// Pass B: raw inputs and statistics (a_r, a_g, a_b, b) per channel.
+// Pass B of a group: raw inputs and statistics (a_r, a_g, a_b, b) per channel.
 template <int G>
 struct PassB {
     static constexpr int NR = 4 * G;
@@ -220,32 +704,28 @@ struct PassB {
 
     __device__ __forceinline__ void init()
     {
-        const double imax = bound_of(a, 0);
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const double pmax = bound_of(a, 1 + a.first_channel + g);
-            qa[g] = make_quant(qexp(7.0 * imax * pmax / a.eps), a.r);
-            qb[g] = make_quant(qexp(pmax + 21.0 * imax * imax * pmax / a.eps), a.r);
-        }
+        for (int g = 0; g < G; ++g) coeff_quants(a, a.first_channel + g, qa[g], qb[g]);
         weight = 1.0 / (double)((2 * a.r + 1) * (2 * a.r + 1));
     }
     __device__ __forceinline__ void raw(long long pix, double* v) const
     {
         const long long n = (long long)a.w * a.h;
+        const double* ab = a.ab + (long long)(4 * a.first_channel) * n;
 #pragma unroll
-        for (int k = 0; k < 4 * G; ++k) v[k] = a.ab[(long long)k * n + pix];
+        for (int k = 0; k < 4 * G; ++k) v[k] = ab[(long long)k * n + pix];
     }
-    __device__ __forceinline__ void stats(const double* v, long long* s) const
+    __device__ __forceinline__ void stats(const double* v, double* s) const
     {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            s[4 * g + 0] = qa[g].q(v[4 * g + 0]);
-            s[4 * g + 1] = qa[g].q(v[4 * g + 1]);
-            s[4 * g + 2] = qa[g].q(v[4 * g + 2]);
-            s[4 * g + 3] = qb[g].q(v[4 * g + 3]);
+            s[4 * g + 0] = qa[g].qc(v[4 * g + 0]);
+            s[4 * g + 1] = qa[g].qc(v[4 * g + 1]);
+            s[4 * g + 2] = qa[g].qc(v[4 * g + 2]);
+            s[4 * g + 3] = qb[g].qc(v[4 * g + 3]);
         }
     }
-    __device__ __forceinline__ void out(long long pix, long long, const long long* H) const
+    __device__ __forceinline__ void out(long long pix, long long, const double* H) const
     {
         const double I0 = a.rgb[3 * pix], I1 = a.rgb[3 * pix + 1], I2 = a.rgb[3 * pix + 2];
 #pragma unroll
@@ -254,34 +734,15 @@ struct PassB {
             q = q + qa[g].mean(H[4 * g + 0], weight) * I0;
             q = q + qa[g].mean(H[4 * g + 1], weight) * I1;
             q = q + qa[g].mean(H[4 * g + 2], weight) * I2;
-            a.q[g][pix] = q;
+            a.q[a.first_channel + g][pix] = q;
         }
     }
 };
 
-// v of the lane the DPP control selects (0 where it selects none), both halves
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ long long dpp64(long long v)
-{
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v & 0xffffffffll), CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROW_MASK, 0xf, false);
-    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-
-// inclusive prefix sum of a 64-bit integer across the 64 lanes of a wave:
-// shifts by 1, 2, 4, 8 within rows of 16 lanes, then the row broadcasts of
-// lanes 15 and 31 (GFX9 DPP)
-__device__ __forceinline__ long long wave_scan(long long v, int)
-{
-    v += dpp64<0x111, 0xf>(v);  // row_shr:1
-    v += dpp64<0x112, 0xf>(v);  // row_shr:2
-    v += dpp64<0x114, 0xf>(v);  // row_shr:4
-    v += dpp64<0x118, 0xf>(v);  // row_shr:8
-    v += dpp64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    return v;
-}
-
+// A workgroup of E threads owns E consecutive (clamped) columns, of which the
+// middle TW = E - 2r are its output columns, and walks RH rows: vertical
+// window sums in registers, a row's horizontal window sums the differences
+// of a workgroup-wide inclusive prefix (wave scans, one LDS exchange).
 template <class Src, int E>
 __global__ void __launch_bounds__(E) strip_kernel(Src src, int TW, int RH)
 {
@@ -290,8 +751,8 @@ __global__ void __launch_bounds__(E) strip_kernel(Src src, int TW, int RH)
     constexpr int NW = E / 64;
     // statistics exchanged through LDS per round (48 KiB of prefixes)
     constexpr int KC = (6144 / E) < NS ? (6144 / E) : NS;
-    __shared__ long long pref[KC][E];
-    __shared__ long long tot[KC][NW];
+    __shared__ double pref[KC][E];
+    __shared__ double tot[KC][NW];
 
     src.init();
     const FilterArgs& a = src.a;
@@ -303,12 +764,11 @@ __global__ void __launch_bounds__(E) strip_kernel(Src src, int TW, int RH)
     const int y0 = blockIdx.y * RH;
     const int y1 = min(a.h, y0 + RH);
 
-    long long V[NS];
+    double V[NS];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) V[k] = 0;
+    for (int k = 0; k < NS; ++k) V[k] = 0.0;
     for (int dy = -r; dy <= r; ++dy) {
-        double v[NR];
-        long long s[NS];
+        double v[NR], s[NS];
         src.raw((long long)clampi(y0 + dy, 0, a.h - 1) * a.w + col, v);
         src.stats(v, s);
 #pragma unroll
@@ -330,14 +790,14 @@ __global__ void __launch_bounds__(E) strip_kernel(Src src, int TW, int RH)
         }
 
         // horizontal window sums: differences of the strip's prefix sums
-        long long H[NS];
+        double H[NS];
 #pragma unroll
         for (int c0 = 0; c0 < NS; c0 += KC) {
-            long long P[KC];
+            double P[KC];
 #pragma unroll
             for (int j = 0; j < KC; ++j) {
                 if (c0 + j < NS) {
-                    P[j] = wave_scan(V[c0 + j], lane);
+                    P[j] = wave_scan(V[c0 + j]);
                     pref[j][t] = P[j];
                     if (lane == 63) tot[j][wave] = P[j];
                 }
@@ -347,7 +807,7 @@ __global__ void __launch_bounds__(E) strip_kernel(Src src, int TW, int RH)
 #pragma unroll
                 for (int j = 0; j < KC; ++j) {
                     if (c0 + j < NS) {
-                        long long own = P[j], lo = 0;
+                        double own = P[j], lo = 0.0;
                         for (int w = 0; w < wave; ++w) own += tot[j][w];
                         if (from >= 0) {
                             lo = pref[j][from];
@@ -362,24 +822,84 @@ __global__ void __launch_bounds__(E) strip_kernel(Src src, int TW, int RH)
         if (writer) src.out((long long)y * a.w + ox, n, H);
 
         if (more) {
-            long long si[NS], so[NS];
+            double si[NS], so[NS];
             src.stats(vin, si);
             src.stats(vout, so);
 #pragma unroll
-            for (int k = 0; k < NS; ++k) V[k] = (V[k] + si[k]) - so[k];
+            for (int k = 0; k < NS; ++k) V[k] = V[k] + (si[k] - so[k]);
         }
     }
+}
+
+// ===========================================================================
+// Launch geometry
+// ===========================================================================
+
+struct WaveGeometry {
+    int K, TW, RH, strips, chunks, tiles;
+};
+
+// One-wave strips: 64 K columns (K = 4 unless block asks for 128 columns and
+// the radius leaves 64 output columns), rows per strip so that the launch has
+// about 4096 channel waves, at least 32; rows overrides it (the autotuner's knob).
+bool wave_geometry(int w, int h, int r, int channels, int block, int rows, WaveGeometry& g)
+{
+    if (r < 1 || r > kWaveStripMaxRadius) return false;
+    g.K = (block == 128 && 128 - 2 * r >= 64) ? 2 : 4;
+    g.TW = 64 * g.K - 2 * r;
+    g.strips = (w + g.TW - 1) / g.TW;
+    int rh;
+    if (rows > 0) {
+        rh = rows;
+    } else {
+        const long long per_row = (long long)g.strips * channels;
+        const long long want = (4096 + per_row - 1) / per_row;
+        rh = (int)((h + want - 1) / want);
+        rh = rh < 32 ? 32 : rh;
+    }
+    g.RH = rh;
+    g.chunks = (h + rh - 1) / rh;
+    g.tiles = g.strips * g.chunks;
+    return true;
+}
+
+constexpr int kTileChannels = 7;  // channel waves of one pass-A workgroup (plus the guidance wave)
+constexpr int kOutChannels = 8;   // channel waves of one pass-B workgroup
+
+template <bool PASS_A, int K, int RHO>
+hipError_t launch_tiles_k(const FilterArgs& a, const WaveGeometry& g, hipStream_t s)
+{
+    const int S = PASS_A ? (a.channels < kTileChannels ? a.channels : kTileChannels)
+                         : (a.channels < kOutChannels ? a.channels : kOutChannels);
+    const int groups = (a.channels + S - 1) / S;
+    const long long blocks = 8ll * ((g.tiles + 7) / 8) * groups;
+    if (blocks > 0x7fffffffll) return hipErrorInvalidValue;
+    if (PASS_A)
+        hipLaunchKernelGGL((coeff_tile_kernel<K, RHO>), dim3((unsigned)blocks), dim3(64 * (1 + S)), 0, s, a, g.TW,
+                           g.RH, g.strips, g.tiles, S);
+    else
+        hipLaunchKernelGGL((output_tile_kernel<K, RHO>), dim3((unsigned)blocks), dim3(64 * S), 0, s, a, g.TW, g.RH,
+                           g.strips, g.tiles, S);
+    return hipGetLastError();
+}
+
+template <bool PASS_A>
+hipError_t launch_tiles(const FilterArgs& a, hipStream_t s)
+{
+    WaveGeometry g;
+    if (!wave_geometry(a.w, a.h, a.r, a.channels, a.block, a.rows, g)) return hipErrorInvalidValue;
+    const int rho = (2 * a.r + 1) % g.K;
+    if (g.K == 2) return launch_tiles_k<PASS_A, 2, 1>(a, g, s);
+    return rho == 1 ? launch_tiles_k<PASS_A, 4, 1>(a, g, s) : launch_tiles_k<PASS_A, 4, 3>(a, g, s);
 }
 
 struct Geometry {
     int E, TW, RH, strips, chunks;
 };
 
-// Strip geometry: workgroup width E (its middle TW = E - 2r columns are
-// output columns) and rows per workgroup RH.  By default E is the smallest
-// width whose output part is at least half of it and RH aims at ~2048
-// workgroups, at least 2r + 2 rows (the vertical warm-up is 2r + 1 rows).
-// block / rows override them (the autotuner's knobs).
+// Workgroup strips: width E (its middle TW = E - 2r columns are output
+// columns), by default the smallest width whose output part is at least half
+// of it, and RH aiming at ~2048 workgroups, at least 2r + 2 rows.
 bool geometry(int w, int h, int r, int block, int rows, Geometry& g)
 {
     if (r < 0 || r > kMaxFilterRadius) return false;
@@ -425,23 +945,33 @@ hipError_t launch_e(const FilterArgs& a, const Geometry& geo, hipStream_t s)
 }
 
 template <bool PASS_A>
-hipError_t launch_pass(const FilterArgs& a, hipStream_t s)
+hipError_t launch_pass(const FilterArgs& args, hipStream_t s)
 {
-    Geometry geo;
-    if (a.w <= 0 || a.h <= 0) return hipSuccess;
-    if (a.groups < 1 || a.groups > kMaxGroup) return hipErrorInvalidValue;
-    if (a.r == 0) {
+    if (args.w <= 0 || args.h <= 0) return hipSuccess;
+    if (args.channels < 1 || args.channels > kMaxChannels) return hipErrorInvalidValue;
+    if (args.r < 0 || args.r > kMaxFilterRadius) return hipErrorInvalidValue;
+    if (args.r == 0) {
         if (PASS_A) return hipSuccess;  // the identity needs no coefficients
-        const long long n = (long long)a.w * a.h;
-        hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+        const long long n = (long long)args.w * args.h;
+        hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, args);
         return hipGetLastError();
     }
-    if (!geometry(a.w, a.h, a.r, a.block, a.rows, geo)) return hipErrorInvalidValue;
-    switch (geo.E) {
-    case 256: return launch_e<PASS_A, 256>(a, geo, s);
-    case 512: return launch_e<PASS_A, 512>(a, geo, s);
-    default: return launch_e<PASS_A, 1024>(a, geo, s);
+    if (args.r <= kWaveStripMaxRadius) return launch_tiles<PASS_A>(args, s);
+    Geometry geo;
+    if (!geometry(args.w, args.h, args.r, args.block, args.rows, geo)) return hipErrorInvalidValue;
+    for (int g0 = 0; g0 < args.channels; g0 += kMaxGroup) {
+        FilterArgs a = args;
+        a.first_channel = g0;
+        a.groups = args.channels - g0 < kMaxGroup ? args.channels - g0 : kMaxGroup;
+        hipError_t e;
+        switch (geo.E) {
+        case 256: e = launch_e<PASS_A, 256>(a, geo, s); break;
+        case 512: e = launch_e<PASS_A, 512>(a, geo, s); break;
+        default: e = launch_e<PASS_A, 1024>(a, geo, s); break;
+        }
+        if (e != hipSuccess) return e;
     }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -451,8 +981,8 @@ hipError_t launch_filter_bounds(const BoundsArgs& args, hipStream_t stream)
     if (args.w <= 0 || args.h <= 0) return hipSuccess;
     if (args.channels < 1 || args.channels > kMaxChannels) return hipErrorInvalidValue;
     const long long n = (long long)args.w * args.h;
-    long long blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    long long blocks = (3 * n + 2047) / 2048;  // at least 4 pairs of the guidance a thread
+    if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(bounds_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, args);
     return hipGetLastError();
 }

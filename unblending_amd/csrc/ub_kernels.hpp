@@ -102,21 +102,26 @@ struct Channel {
     int stride;
 };
 
-constexpr int kMaxGroup = 3;      // channels filtered by one strip launch
+constexpr int kMaxGroup = 3;      // channels of one launch of the wide strip kernel (r > 96)
 constexpr int kMaxChannels = 16;  // channels of one filter call (L alphas + 3 background)
+constexpr int kWaveStripMaxRadius = 96;  // radii filtered by the one-wave strip kernel
 
 struct FilterArgs {
     const double* rgb;    // guidance [n][3]
     int w, h, r;
     double eps;
-    int groups;           // channels in this launch (<= kMaxGroup)
-    int first_channel;    // index of this launch's first channel in the call (its bounds slot - 1)
-    Channel in[kMaxGroup];// pass A inputs
-    double* ab;           // pass A output / pass B input: [4 * group][n] (a_r, a_g, a_b, b)
-    double* q[kMaxGroup]; // pass B outputs
+    double isr;           // 1 / sqrt(eps), the coefficient bounds' factor (computed on the host)
+    int channels;         // channels of the call
+    int first_channel;    // wide strip kernel: this launch's first channel
+    int groups;           // wide strip kernel: channels in this launch (<= kMaxGroup)
+    Channel in[kMaxChannels];  // pass A inputs
+    double* ab;           // pass A output / pass B input: [4 * channels][n] (a_r, a_g, a_b, b per channel)
+    int group;            // r <= 96: channels per wave of pass A (1 to 3; 0: 1)
+    double* q[kMaxChannels];   // pass B outputs
     const unsigned long long* bounds;  // max|I| then max|p_c| of the call's channels (bit patterns)
-    int block;            // workgroup width (256, 512 or 1024; 0: chosen by radius)
-    int rows;             // rows per workgroup (0: chosen by image size)
+    int block;            // r <= 96: strip width in columns (128 or 256; 0: 256);
+                          // r > 96: workgroup width (256, 512 or 1024; 0: chosen by radius)
+    int rows;             // rows per strip (0: chosen by image size)
 };
 
 // max|I| and max|p_c| of every channel of a filter call: bounds[0] and
@@ -129,9 +134,9 @@ struct BoundsArgs {
     unsigned long long* bounds;
 };
 
-// The exact-sum filter (ub_filter.hip): the bounds pass, then per group of
-// channels pass A (box statistics and per-pixel coefficients) and pass B (box
-// of the coefficients and the output q).  The strip launches return
+// The exact-sum filter (ub_filter.hip): the bounds pass, then pass A (box
+// statistics and per-pixel coefficients of every channel) and pass B (box of
+// the coefficients and the output q of every channel).  The passes return
 // hipErrorInvalidValue when the radius is beyond their reach
 // (r > kMaxFilterRadius).
 constexpr int kMaxFilterRadius = 480;

@@ -2,13 +2,16 @@
 //
 // First pass (unblending/src/unblending.cpp:441-475): one solve launch.
 // Refinement (unblending.cpp:335-439): guided filter of every alpha (and the
-// background colour when smoothing) in groups of up to kMaxGroup channels --
-// pass A then pass B per group -- into filtered planes that the re-solve
-// launch reads directly (crop and normalisation happen in its prologue).
+// background colour when smoothing) -- a bounds pass, pass A (coefficients of
+// every channel), pass B (outputs of every channel) -- into filtered planes
+// that the re-solve launch reads directly (crop and normalisation happen in
+// its prologue).
 // The host-pointer path (host_solve) adds the uploads and the download, as
 // the reference's solve_cuda does (pixel_pipeline.cu:63-105), on a stream of
 // its own, and returns after synchronising it.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -119,14 +122,13 @@ void require_device()
 
 }  // namespace
 
-// filter workspace: the bounds slots, then one group's coefficient planes
+// filter workspace: the bounds slots, then every channel's coefficient planes
 constexpr size_t kBoundsBytes = 256;
 static_assert(kBoundsBytes >= (1 + kMaxChannels) * sizeof(unsigned long long), "bounds slots");
 
 size_t filter_workspace_bytes(size_t n, int channels)
 {
-    const size_t group = size_t(std::min(channels, kMaxGroup));
-    return kBoundsBytes + align256(4 * group * n * sizeof(double));
+    return kBoundsBytes + align256(4 * size_t(channels) * n * sizeof(double));
 }
 
 namespace {
@@ -230,34 +232,34 @@ void run_filter(const double* rgb, int w, int h, const Channel* in, double* cons
         launches += 2;
         if (timing) coeff_ms += ea.ms();
     }
-    for (int g0 = 0; g0 < channels; g0 += kMaxGroup) {
-        FilterArgs fa{};
-        fa.rgb = rgb;
-        fa.w = w;
-        fa.h = h;
-        fa.r = radius;
-        fa.eps = eps;
-        fa.groups = std::min(kMaxGroup, channels - g0);
-        fa.first_channel = g0;
-        for (int j = 0; j < fa.groups; ++j) {
-            fa.in[j] = in[g0 + j];
-            fa.q[j] = q[g0 + j];
-        }
-        fa.ab = planes;
-        fa.bounds = bounds;
-        fa.block = block;
-        fa.rows = rows;
-        ea.start(stream);
-        check(launch_filter_coefficients(fa, stream), "guided filter (coefficients)");
-        ea.stop(stream);
-        eb.start(stream);
-        check(launch_filter_output(fa, stream), "guided filter (output)");
-        eb.stop(stream);
-        launches += 2;
-        if (timing) {
-            coeff_ms += ea.ms();
-            out_ms += eb.ms();
-        }
+    FilterArgs fa{};
+    fa.rgb = rgb;
+    fa.w = w;
+    fa.h = h;
+    fa.r = radius;
+    fa.eps = eps;
+    fa.isr = 1.0 / std::sqrt(eps);
+    fa.channels = channels;
+    for (int c = 0; c < channels; ++c) {
+        fa.in[c] = in[c];
+        fa.q[c] = q[c];
+    }
+    fa.ab = planes;
+    fa.group = 1;
+    if (const char* g = std::getenv("UNBLENDING_FILTER_GROUP")) fa.group = std::atoi(g);
+    fa.bounds = bounds;
+    fa.block = block;
+    fa.rows = rows;
+    ea.start(stream);
+    check(launch_filter_coefficients(fa, stream), "guided filter (coefficients)");
+    ea.stop(stream);
+    eb.start(stream);
+    check(launch_filter_output(fa, stream), "guided filter (output)");
+    eb.stop(stream);
+    launches += 2;
+    if (timing) {
+        coeff_ms += ea.ms();
+        out_ms += eb.ms();
     }
     if (stats) {
         stats->filter_coeff_ms = coeff_ms;
