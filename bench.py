@@ -408,6 +408,9 @@ def main():
                   "unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
                   "unmix_evals_per_px": s1.evals / n, "unmix_iters_per_px": s1.iterations / n,
                   "unmix_outer_per_px": s1.outer / n, "refine_evals_per_px": s2.evals / n,
+                  "refine_iters_per_px": s2.iterations / n, "refine_outer_per_px": s2.outer / n,
+                  "unmix_lane_use": s1.evals / s1.lane_slots if s1.lane_slots else None,
+                  "refine_lane_use": s2.evals / s2.lane_slots if s2.lane_slots else None,
                   "unmix_tflops": round(flops1 / (s1.solve_ms * 1e-3) / 1e12, 3),
                   "refine_tflops": round(flops2 / (s2.solve_ms * 1e-3) / 1e12, 3)},
     }

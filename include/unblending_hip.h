@@ -71,6 +71,8 @@ typedef struct unblending_hip_stats {
     int32_t reserved;
     uint64_t fallback_pixels; /* warm start: pixels solved again from the default initialisation */
     double predict_ms;        /* warm start: device time of the model's prediction (timing = 1) */
+    uint64_t lane_slots;      /* persistent solve: lane evaluations, useful or idle (evals / lane_slots
+                                 is the share of lanes doing useful work) */
 } unblending_hip_stats;
 
 UNBLENDING_API int32_t unblending_hip_unmix_ex(

@@ -168,7 +168,8 @@ class HipStats(ctypes.Structure):
                 ("filter_coeff_ms", ctypes.c_double), ("filter_out_ms", ctypes.c_double),
                 ("filter_radius", ctypes.c_int32), ("filter_channels", ctypes.c_int32),
                 ("filter_launches", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("fallback_pixels", ctypes.c_uint64), ("predict_ms", ctypes.c_double)]
+                ("fallback_pixels", ctypes.c_uint64), ("predict_ms", ctypes.c_double),
+                ("lane_slots", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

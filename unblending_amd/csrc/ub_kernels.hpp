@@ -30,8 +30,10 @@ struct SolverParams {
 // kQueue / kQueue2 are the persistent solve's next-item counters; kFallback
 // counts the pixels whose warm-started attempt missed the tolerances and
 // that were solved again from the default start.
+// kSlots counts the persistent solve's lane evaluations, useful or idle
+// (trips of its loop times 64).
 enum Counter : int {
-    kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6,
+    kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6, kSlots = 7,
     kGroupQueue = 8,  // batches: the queue of the launch of the g-th layer count is kGroupQueue + g
     kNumCounters = 16
 };
@@ -116,7 +118,6 @@ struct FilterArgs {
     int groups;           // wide strip kernel: channels in this launch (<= kMaxGroup)
     Channel in[kMaxChannels];  // pass A inputs
     double* ab;           // pass A output / pass B input: [4 * channels][n] (a_r, a_g, a_b, b per channel)
-    int group;            // r <= 96: channels per wave of pass A (1 to 3; 0: 1)
     double* q[kMaxChannels];   // pass B outputs
     const unsigned long long* bounds;  // max|I| then max|p_c| of the call's channels (bit patterns)
     int block;            // r <= 96: strip width in columns (128 or 256; 0: 256);

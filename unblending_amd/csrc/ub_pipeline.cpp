@@ -11,7 +11,6 @@
 // its own, and returns after synchronising it.
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -245,8 +244,6 @@ void run_filter(const double* rgb, int w, int h, const Channel* in, double* cons
         fa.q[c] = q[c];
     }
     fa.ab = planes;
-    fa.group = 1;
-    if (const char* g = std::getenv("UNBLENDING_FILTER_GROUP")) fa.group = std::atoi(g);
     fa.bounds = bounds;
     fa.block = block;
     fa.rows = rows;
@@ -381,6 +378,7 @@ void run_device(const DeviceJob& job)
         job.stats->outer = host_counters[kOuter];
         job.stats->pixels = host_counters[kPixels];
         job.stats->fallback_pixels = host_counters[kFallback];
+        job.stats->lane_slots = host_counters[kSlots];
         job.stats->solve_ms = es.ms();
         job.stats->predict_ms = predict_ms;
     }
@@ -547,6 +545,7 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
         stats->iterations = host_counters[kIterations];
         stats->outer = host_counters[kOuter];
         stats->pixels = host_counters[kPixels];
+        stats->lane_slots = host_counters[kSlots];
         stats->solve_ms = es.ms();
     }
 }

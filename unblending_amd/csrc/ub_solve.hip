@@ -462,7 +462,8 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // Only the outer iterate, read and written once per outer iteration, has a
 // per-lane global slot.
 // ---------------------------------------------------------------------------
-enum : int { kNeedPixel = 0, kStart = 1, kTrial = 2, kWait = 3, kOuter = 4, kRetry = 5 };
+// per-lane phases of the persistent solve (distinct from the Counter slots)
+enum Phase : int { kPhNeedPixel = 0, kPhStart = 1, kPhTrial = 2, kPhWait = 3, kPhOuter = 4, kPhRetry = 5 };
 
 // Lanes waiting for a pixel, or for their outer update, are served together
 // once at least this many wait (or nothing else is left to do).
@@ -538,7 +539,7 @@ struct WaveQueue {
 };
 
 struct LaneCounters {
-    unsigned long long evals, iters, outer, pixels;
+    unsigned long long evals, iters, outer, pixels, slots;
 };
 
 // Claims the next non-empty chunk of the queue: its image and item range.
@@ -599,7 +600,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     double xc[N], S[N], Y[N];
     double rh = 0.0;
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
-    int phase = kNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
+    int phase = kPhNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
     bool warm_try = false;  // this pixel's solve started from the warm-start prediction
     long long pix = 0;
 #pragma unroll
@@ -651,10 +652,10 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         // (handed out in batches of at least kFetchBatch lanes while others are
         // still solving: the start of a pixel is a long stretch of code that the
         // whole wave executes whenever any lane needs it)
-        unsigned long long needy = __ballot(phase == kNeedPixel);
-        const unsigned long long retry = __ballot(phase == kRetry);
+        unsigned long long needy = __ballot(phase == kPhNeedPixel);
+        const unsigned long long retry = __ballot(phase == kPhRetry);
         const bool serve = __popcll(needy | retry) >= kFetchBatch ||
-                           __ballot(phase == kStart || phase == kTrial || phase == kOuter) == 0;
+                           __ballot(phase == kPhStart || phase == kPhTrial || phase == kPhOuter) == 0;
         if (!serve) needy = 0;
         long long item = -1;
         while (needy) {
@@ -685,11 +686,11 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             q.next += __popcll(took);
             needy &= ~took;
         }
-        if (item >= 0 || (serve && phase == kRetry)) {
+        if (item >= 0 || (serve && phase == kPhRetry)) {
             // a new pixel starts from the warm-start prediction when there is
             // one; a retried pixel from the default start
-            if (phase != kRetry) pix = dd->list ? dd->list[item] : item;
-            warm_try = !REFINE && dd->warm != nullptr && phase != kRetry;
+            if (phase != kPhRetry) pix = dd->list ? dd->list[item] : item;
+            warm_try = !REFINE && dd->warm != nullptr && phase != kPhRetry;
             double x0[N], C0[NC], tg0[3], ta0[L];
             load_pixel<L, REFINE>(*dd, lay, pix, tg0, ta0, bg, x0, warm_try);
 #pragma unroll
@@ -711,13 +712,13 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 bounds(i, lo, hi);
                 S[i] = clampd(x0[i], lo, hi);
             }
-            phase = kStart;
-        } else if (phase == kNeedPixel && (q.pending || q.exhausted)) {
-            phase = kWait;
+            phase = kPhStart;
+        } else if (phase == kPhNeedPixel && (q.pending || q.exhausted)) {
+            phase = kPhWait;
         }
 
         // ---- a drained wave moves to its pending chunk, or ends ----
-        if (__ballot(phase == kStart || phase == kTrial || phase == kOuter || phase == kRetry) == 0) {
+        if (__ballot(phase == kPhStart || phase == kPhTrial || phase == kPhOuter || phase == kPhRetry) == 0) {
             if (!q.pending) {
                 q.img = -1;
                 return;
@@ -726,16 +727,17 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             q.next = q.pnext;
             q.end = q.pend;
             q.pending = false;
-            phase = kNeedPixel;
+            phase = kPhNeedPixel;
             continue;
         }
 
         // ---- one evaluation per lane at the trial point S ----
+        ++lc.slots;
         double Ct[PS::NCC];
         const double ft = objective<L, REFINE>(S, lay, pc.tgt, pc.ta, lam, rho, Y, Ct);
 
         bool direction = false, inner_end = false;
-        if (phase == kStart) {
+        if (phase == kPhStart) {
             nev = 1;
             f = ft;
 #pragma unroll
@@ -746,7 +748,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
 #pragma unroll
             for (int j = 0; j < PS::NCC; ++j) cc[j] = Ct[j];
             direction = true;
-        } else if (phase == kTrial) {
+        } else if (phase == kPhTrial) {
             ++nev;
             double dphi = 0.0;
 #pragma unroll
@@ -875,15 +877,15 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                     bounds(i, lo, hi);
                     S[i] = clampd(xc[i] + t * q[i], lo, hi);
                 }
-                phase = kTrial;
+                phase = kPhTrial;
             }
         }
 
-        if (inner_end) phase = kOuter;
-        const unsigned long long outer_lanes = __ballot(phase == kOuter);
+        if (inner_end) phase = kPhOuter;
+        const unsigned long long outer_lanes = __ballot(phase == kPhOuter);
         const bool run_outer = __popcll(outer_lanes) >= kOuterBatch ||
-                               __ballot(phase == kStart || phase == kTrial) == 0;
-        if (run_outer && phase == kOuter) {
+                               __ballot(phase == kPhStart || phase == kPhTrial) == 0;
+        if (run_outer && phase == kPhOuter) {
             lc.evals += (unsigned long long)nev;
             lc.outer += 1;
             // the constraint vector at the current point: the composite part
@@ -915,7 +917,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 // tolerances (|C| < eps and step < eps) by its outer cap: the
                 // pixel is solved again from the default start
                 if (dd->fallback_count) atomicAdd(dd->fallback_count, 1ull);
-                phase = kRetry;
+                phase = kPhRetry;
             } else if (converged || ocount > sp.max_count) {
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
@@ -927,7 +929,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                     if (dd->alpha_planes) dd->alpha_planes[(long long)l * dd->n + pix] = xc[l];
                 }
                 ++lc.pixels;
-                phase = kNeedPixel;
+                phase = kPhNeedPixel;
             } else {
                 ++ocount;
                 count = 0;
@@ -938,7 +940,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                     bounds(i, lo, hi);
                     S[i] = clampd(xc[i], lo, hi);
                 }
-                phase = kStart;
+                phase = kPhStart;
             }
         }
     }
@@ -948,7 +950,7 @@ template <int L, bool REFINE>
 __global__ void __launch_bounds__(64) solve_batch_kernel(BatchArgs args)
 {
     __shared__ double smem[persistent_lds_doubles<L, REFINE>() * 64];
-    LaneCounters lc{0, 0, 0, 0};
+    LaneCounters lc{0, 0, 0, 0, 0};
     WaveQueue q{};
     q.img = -1;
     int i0;
@@ -965,12 +967,14 @@ __global__ void __launch_bounds__(64) solve_batch_kernel(BatchArgs args)
         lc.iters += __shfl_xor(lc.iters, off);
         lc.outer += __shfl_xor(lc.outer, off);
         lc.pixels += __shfl_xor(lc.pixels, off);
+        lc.slots += __shfl_xor(lc.slots, off);
     }
     if (threadIdx.x == 0) {
         atomicAdd(&args.counters[kEvals], lc.evals);
         atomicAdd(&args.counters[kIterations], lc.iters);
         atomicAdd(&args.counters[kOuter], lc.outer);
         atomicAdd(&args.counters[kPixels], lc.pixels);
+        atomicAdd(&args.counters[kSlots], lc.slots);
     }
 }
 
