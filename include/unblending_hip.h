@@ -180,6 +180,23 @@ UNBLENDING_API int32_t unblending_hip_refine_batch_device(
     const unblending_hip_params* params, void* workspace, size_t workspace_size, void* stream,
     unblending_hip_stats* stats);
 
+/* ---- image-level recomposite ----
+ * Replaces the reference's CPU composite_layers(layers, comp_ops, modes)
+ * (unblending/include/unblending/unblending.hpp:34-36,
+ * unblending/src/unblending.cpp:638-664): the packed straight-alpha layers
+ * [layer_count][height][width][RGBA] (bottom layer first) folded bottom to
+ * top with each layer's Porter-Duff factors comp_ops[3 l .. 3 l + 2]
+ * (X, Y, Z) and blend mode modes[l] (equations.cpp:10-74, no crop), into
+ * output [height][width][RGBA].  1 to 16 layers.  The _device form takes
+ * device pointers and is enqueued on `stream`; the host form copies in and
+ * out and returns after the result is written. */
+UNBLENDING_API int32_t unblending_hip_composite_device(
+    const double* layers, int32_t width, int32_t height, size_t layer_count, const int32_t* comp_ops,
+    const int32_t* modes, double* output, void* stream);
+UNBLENDING_API int32_t unblending_hip_composite(
+    const double* layers, size_t layers_size, int32_t width, int32_t height, size_t layer_count,
+    const int32_t* comp_ops, const int32_t* modes, double* output, size_t output_capacity);
+
 /* Planar seam, host pointers (see above).  Returns an unblending status. */
 UNBLENDING_API int32_t uhip_solve_planes(
     int32_t width, int32_t height, const unblending_layer* layers, size_t layer_count,

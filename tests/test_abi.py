@@ -274,3 +274,27 @@ def test_batch_entry_points_validate():
     nine, keep9 = _batch_images([(4, 3)], [9])
     assert unmix(nine, 1, ctypes.byref(opt), None, buf, 1 << 40, None, None) == U.DEVICE_ERROR
     assert "1 to 8 layers" in U.last_error()
+
+
+def test_composite_argument_checks_before_any_device_work():
+    """unblending_hip_composite: argument errors and the output capacity are
+    checked on the host, before the device is touched."""
+    lib = U.lib()
+    L, h, w = 2, 3, 4
+    packed = np.zeros((L, h, w, 4))
+    out = np.zeros((h, w, 4))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ops = (ctypes.c_int32 * 6)(1, 1, 1, 1, 1, 1)
+
+    def call(modes, size=packed.size, cap=out.size, n=L, width=w, p=packed):
+        md = (ctypes.c_int32 * len(modes))(*modes)
+        return lib.unblending_hip_composite(p.ctypes.data_as(dp) if p is not None else None, size, width, h, n,
+                                            ops, md, out.ctypes.data_as(dp), cap)
+
+    assert call([0, 13]) == U.INVALID_ARGUMENT          # unknown blend mode
+    assert call([0, -1]) == U.INVALID_ARGUMENT
+    assert call([0, 1], size=packed.size - 1) == U.INVALID_ARGUMENT  # not the packed size
+    assert call([0, 1], cap=out.size - 1) == U.BUFFER_TOO_SMALL
+    assert call([0, 1], width=0) == U.INVALID_ARGUMENT
+    assert call([0, 1], p=None) == U.INVALID_ARGUMENT
+    assert call([0] * 17, n=17, size=17 * h * w * 4) == U.INVALID_ARGUMENT  # more than 16 layers

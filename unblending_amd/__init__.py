@@ -247,6 +247,16 @@ for _f in (_lib.unblending_hip_unmix_batch_device, _lib.unblending_hip_refine_ba
 _lib.unblending_hip_last_error.restype = ctypes.c_char_p
 
 
+_lib.unblending_hip_composite_device.restype = ctypes.c_int32
+_lib.unblending_hip_composite_device.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_size_t,
+                                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                                 _vp, _vp]
+_lib.unblending_hip_composite.restype = ctypes.c_int32
+_lib.unblending_hip_composite.argtypes = [_dp, ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _dp,
+                                          ctypes.c_size_t]
+
+
 def lib():
     return _lib
 
@@ -370,17 +380,31 @@ def perform_matte_refinement(image, layers, layer_infos, has_opaque_background, 
     return out
 
 
-def composite_layers(packed, layer_infos):
-    """Image-level recomposite (unblending/src/unblending.cpp:638-664), numpy."""
-    from . import synth
-    packed = np.asarray(packed, dtype=np.float64)
-    alphas = [packed[l, ..., 3] for l in range(len(layer_infos))]
-    colors = [packed[l, ..., :3] for l in range(len(layer_infos))]
-    models = [synth.LayerModel(int(i.blend_mode), i.color_model.mu, i.color_model.sigma_inv) for i in layer_infos]
-    if not all(i.comp_op.is_source_over() for i in layer_infos):
-        raise ValueError("composite_layers supports source-over stacks")
-    color, alpha = synth.composite(alphas, colors, models)
-    return np.concatenate([color, alpha[..., None]], axis=-1)
+def _composite_tables(comp_ops, modes):
+    ops = (ctypes.c_int32 * (3 * len(comp_ops)))(*[v for op in comp_ops for v in (op.X, op.Y, op.Z)])
+    md = (ctypes.c_int32 * len(modes))(*[int(m) for m in modes])
+    return ops, md
+
+
+def composite_layers(packed, comp_ops, modes=None):
+    """Image-level recomposite (unblending/src/unblending.cpp:638-664) on the
+    device: packed (L, h, w, 4) straight-alpha layers, bottom first, folded
+    with each layer's comp-op and blend mode; returns (h, w, 4) RGBA.
+    comp_ops may also be the list of LayerInfo (modes then taken from it)."""
+    if modes is None:
+        infos = comp_ops
+        comp_ops = [i.comp_op for i in infos]
+        modes = [i.blend_mode for i in infos]
+    packed = np.ascontiguousarray(packed, dtype=np.float64)
+    L, h, w, _ = packed.shape
+    if len(comp_ops) != L or len(modes) != L:
+        raise ValueError("one comp-op and one blend mode per layer")
+    out = np.empty((h, w, 4), dtype=np.float64)
+    ops, md = _composite_tables(comp_ops, modes)
+    raise_for_status(_lib.unblending_hip_composite(
+        packed.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), packed.size, w, h, L, ops, md,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+    return out
 
 
 from . import device  # noqa: E402  (device-resident API over torch tensors)

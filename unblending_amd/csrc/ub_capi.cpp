@@ -470,6 +470,62 @@ UNBLENDING_API int32_t unblending_hip_guided_filter_device(const double* rgb, in
     }
 }
 
+// Arguments of the recomposite entries: dimensions, 1 to 16 layers, comp-op
+// factors and blend modes of the reference's enums; the sizes in doubles.
+static int32_t composite_args_ok(const double* layers, int32_t width, int32_t height, size_t count,
+                                 const int32_t* comp_ops, const int32_t* modes, const double* output)
+{
+    if (!layers || !output || !comp_ops || !modes || width <= 0 || height <= 0 || count < 1 ||
+        count > size_t(kMaxCompositeLayers) || width > std::numeric_limits<int>::max() / height)
+        return UNBLENDING_INVALID_ARGUMENT;
+    for (size_t k = 0; k < count; ++k)
+        if (modes[k] < UNBLENDING_NORMAL || modes[k] > UNBLENDING_LINEAR_DODGE) return UNBLENDING_INVALID_ARGUMENT;
+    return UNBLENDING_OK;
+}
+
+UNBLENDING_API int32_t unblending_hip_composite_device(const double* layers, int32_t width, int32_t height,
+                                                       size_t layer_count, const int32_t* comp_ops,
+                                                       const int32_t* modes, double* output, void* stream)
+{
+    g_last_error.clear();
+    const int32_t st = composite_args_ok(layers, width, height, layer_count, comp_ops, modes, output);
+    if (st != UNBLENDING_OK) return st;
+    try {
+        run_composite(layers, width, height, layer_count, comp_ops, modes, output, static_cast<hipStream_t>(stream));
+        return UNBLENDING_OK;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
+UNBLENDING_API int32_t unblending_hip_composite(const double* layers, size_t layers_size, int32_t width,
+                                                int32_t height, size_t layer_count, const int32_t* comp_ops,
+                                                const int32_t* modes, double* output, size_t output_capacity)
+{
+    g_last_error.clear();
+    const int32_t st = composite_args_ok(layers, width, height, layer_count, comp_ops, modes, output);
+    if (st != UNBLENDING_OK) return st;
+    const size_t n = size_t(width) * size_t(height);
+    if (layers_size != 4 * n * layer_count) return UNBLENDING_INVALID_ARGUMENT;
+    if (output_capacity < 4 * n) return UNBLENDING_BUFFER_TOO_SMALL;
+    try {
+        host_composite(layers, width, height, layer_count, comp_ops, modes, output);
+        return UNBLENDING_OK;
+    } catch (const std::bad_alloc&) {
+        return UNBLENDING_OUT_OF_MEMORY;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
 UNBLENDING_API int32_t uhip_solve_planes(int32_t width, int32_t height, const unblending_layer* layers,
                                          size_t layer_count, const double* rgb_planes, const double* first_planes,
                                          int32_t opaque, int32_t smooth, double* out_planes)

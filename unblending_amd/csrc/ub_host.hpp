@@ -69,6 +69,11 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
 void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const double* first, bool opaque,
                 bool smooth, const unblending_hip_params& params, double* output, unblending_hip_stats* stats);
 
+void run_composite(const double* layers, int w, int h, size_t count, const int32_t* comp_ops, const int32_t* modes,
+                   double* out, hipStream_t stream);
+void host_composite(const double* layers, int w, int h, size_t count, const int32_t* comp_ops, const int32_t* modes,
+                    double* out);
+
 hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream);
 hipError_t launch_store_desc(ImageDesc* dst, const ImageDesc& d, hipStream_t stream);
 

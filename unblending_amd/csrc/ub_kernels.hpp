@@ -145,4 +145,21 @@ hipError_t launch_filter_bounds(const BoundsArgs& args, hipStream_t stream);
 hipError_t launch_filter_coefficients(const FilterArgs& args, hipStream_t stream);
 hipError_t launch_filter_output(const FilterArgs& args, hipStream_t stream);
 
+// Image-level recomposite (ub_composite.hip): packed [count][n][RGBA] layers
+// folded bottom to top into [n][RGBA].
+constexpr int kMaxCompositeLayers = 16;
+struct CompositeLayer {
+    double X, Y, Z;
+    int mode;
+    int pad;
+};
+struct CompositeArgs {
+    CompositeLayer table[kMaxCompositeLayers];
+    int count;
+    const double* layers;
+    double* out;
+    long long n;
+};
+hipError_t launch_composite(const CompositeArgs& args, hipStream_t stream);
+
 }  // namespace ubhip

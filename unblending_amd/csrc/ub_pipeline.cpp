@@ -591,4 +591,38 @@ void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const doubl
     check(hipStreamSynchronize(stream.s), "download");
 }
 
+// Image-level recomposite (unblending_hip_composite*): comp_ops [count][3]
+// and modes [count] are validated by the caller.
+void run_composite(const double* layers, int w, int h, size_t count, const int32_t* comp_ops, const int32_t* modes,
+                   double* out, hipStream_t stream)
+{
+    CompositeArgs a{};
+    for (size_t k = 0; k < count; ++k) {
+        a.table[k].X = double(comp_ops[3 * k]);
+        a.table[k].Y = double(comp_ops[3 * k + 1]);
+        a.table[k].Z = double(comp_ops[3 * k + 2]);
+        a.table[k].mode = modes[k];
+    }
+    a.count = int(count);
+    a.layers = layers;
+    a.out = out;
+    a.n = (long long)w * h;
+    check(launch_composite(a, stream), "composite");
+}
+
+void host_composite(const double* layers, int w, int h, size_t count, const int32_t* comp_ops, const int32_t* modes,
+                    double* out)
+{
+    require_device();
+    const size_t n = size_t(w) * size_t(h);
+    Stream stream;
+    DeviceBuffer d_in(4 * count * n * sizeof(double));
+    DeviceBuffer d_out(4 * n * sizeof(double));
+    check(hipMemcpyAsync(d_in.p, layers, 4 * count * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
+          "upload layers");
+    run_composite(d_in.as<double>(), w, h, count, comp_ops, modes, d_out.as<double>(), stream.s);
+    check(hipMemcpyAsync(out, d_out.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream.s), "download");
+    check(hipStreamSynchronize(stream.s), "composite");
+}
+
 }  // namespace ubhip

@@ -12,8 +12,8 @@ from __future__ import annotations
 
 import ctypes
 
-from . import (BACKEND_HIP, HipImage, HipParams, HipStats, _lib, hip_params, layer_array, options,
-               raise_for_status)
+from . import (BACKEND_HIP, HipImage, HipParams, HipStats, _composite_tables, _lib, hip_params, layer_array,
+               options, raise_for_status)
 
 
 def _torch():
@@ -119,6 +119,21 @@ def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stat
         ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream_ptr(stream),
         ctypes.byref(stats) if stats is not None else None)
     raise_for_status(st)
+    return out
+
+
+def composite(layers, layer_infos, out=None, stream=None):
+    """Image-level recomposite of device layers (L, h, w, 4) with the layer
+    infos' comp-ops and blend modes into (h, w, 4) RGBA, on the device."""
+    torch = _torch()
+    L, h, w, _ = layers.shape
+    _check(layers, (L, h, w, 4), "layers")
+    if out is None:
+        out = torch.empty((h, w, 4), dtype=torch.float64, device=layers.device)
+    _check(out, (h, w, 4), "out")
+    ops, md = _composite_tables([i.comp_op for i in layer_infos], [i.blend_mode for i in layer_infos])
+    raise_for_status(_lib.unblending_hip_composite_device(ctypes.c_void_p(layers.data_ptr()), w, h, L, ops, md,
+                                                          ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)))
     return out
 
 
