@@ -318,6 +318,16 @@ def main():
     s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape), out)
     first_plain = first.clone()
 
+    def recon(packed):
+        """RMS of (recomposite RGB - image) over the whole image, alpha error
+        max |A - 1|, and the recomposite itself (device recomposite)."""
+        rgba = device.composite(packed, infos)
+        err = rgba[..., :3] - rgb
+        return {"rgb_rms": float(err.square().mean().sqrt()), "alpha_max_err": float((rgba[..., 3] - 1.0).abs().max())}, rgba
+
+    recon_plain_first, rc_first = recon(first_plain)
+    recon_plain_refined, rc_refined = recon(out)
+
     warm = None
     if not args.no_warm:
         from unblending_amd import mlp
@@ -348,6 +358,12 @@ def main():
         out_warm = torch.empty_like(out)
         ms_warm = timed(wparams, out_warm, "warm")
         w1, w2 = instrumented(wparams, out_warm)
+        recon_warm_first, rw_first = recon(first)
+        recon_warm_refined, rw_refined = recon(out_warm)
+        warm_recon = {"first": recon_warm_first, "refined": recon_warm_refined,
+                      "vs_plain_first_rms": float((rw_first - rc_first).square().mean().sqrt()),
+                      "vs_plain_refined_rms": float((rw_refined - rc_refined).square().mean().sqrt())}
+        del rw_first, rw_refined
         d1 = (first - first_plain).abs()
         d2 = (out_warm - out).abs()
         per_train = train_s / max(args.train_steps, 1)
@@ -355,7 +371,7 @@ def main():
         qp = first_pass_quality(first_plain[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
         qw = first_pass_quality(first[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
         warm = {"ms_per_step": ms_warm, "s1": w1, "s2": w2, "train_s": train_s, "label_s": label_s,
-                "quality": {"rows": band, "plain": qp, "warm": qw},
+                "quality": {"rows": band, "plain": qp, "warm": qw}, "recomposite": warm_recon,
                 "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-100:])),
                 "train_tflops": mlp_train_flops(4096) / per_train / 1e12 if per_train > 0 else 0.0,
                 "first_max": float(d1.max()), "first_mean": float(d1.mean()),
@@ -412,7 +428,8 @@ def main():
                   "unmix_lane_use": s1.evals / s1.lane_slots if s1.lane_slots else None,
                   "refine_lane_use": s2.evals / s2.lane_slots if s2.lane_slots else None,
                   "unmix_tflops": round(flops1 / (s1.solve_ms * 1e-3) / 1e12, 3),
-                  "refine_tflops": round(flops2 / (s2.solve_ms * 1e-3) / 1e12, 3)},
+                  "refine_tflops": round(flops2 / (s2.solve_ms * 1e-3) / 1e12, 3),
+                  "recomposite": {"first": recon_plain_first, "refined": recon_plain_refined}},
     }
     if warm:
         w1, w2 = warm["s1"], warm["s2"]
@@ -430,6 +447,7 @@ def main():
             "vs_plain_refined": {"max": warm["refined_max"], "mean": warm["refined_mean"],
                                  "frac_gt_1e-3": warm["refined_frac_gt_1e-3"]},
             "first_pass_quality": warm["quality"],
+            "recomposite": warm["recomposite"],
             "training": {"images": args.train_images, "size": args.train_size, "steps": args.train_steps,
                          "batch": 4096, "lr": args.train_lr, "label_s": round(warm["label_s"], 3),
                          "train_s": round(warm["train_s"], 3), "loss_first": warm["loss_first"],
