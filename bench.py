@@ -8,11 +8,11 @@ reference's initial point) and with the warm-start network, trained first on
 2^20 pixels of other images of the same layer set solved by the plain GPU
 path (Adam, batch 4096, 20k steps; the training time is reported, not
 counted in the step).  Prints one JSON line: `value` is the warm-started
-Mpixel/s, with the plain rate, the difference between the two outputs, the
-roofline of the dominant kernel (FP64 solve), the filter's HBM rate, the
-network's matrix-core rate, and the oracle timed on the host cores on a band
-of the same image (whose first pass and refinement both runs are compared
-with).
+Mpixel/s, with the plain rate, the difference between the two outputs and
+their full-image reconstruction errors (device recomposite), the roofline of
+the dominant kernel (FP64 solve), the filter's HBM rate, the network's
+matrix-core rate, and the oracle timed on the host cores on a band of the
+same image (whose first pass and refinement both runs are compared with).
 """
 from __future__ import annotations
 

@@ -39,7 +39,7 @@ enum Counter : int {
 };
 
 struct PixelArgs {
-    DevLayer layers[kMaxLayers];  // passed by value; staged into LDS by the kernel
+    DevLayer layers[kMaxLayers];  // passed by value; read by the kernel with scalar loads
     const double* rgb;     // [n][3] target colours
     long long n;           // pixels
     SolverParams sp;

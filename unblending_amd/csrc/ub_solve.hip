@@ -8,7 +8,9 @@
 //   free set -> two-loop recursion -> projected Armijo backtracking with
 //   quadratic interpolation in [0.1 t, 0.5 t] -> (s, y) update with a
 //   curvature test -> NLopt's relative ftol / xtol tests.
-// The layer set is staged once per workgroup in LDS and read as a broadcast.
+// The layer set is wave-uniform and read with scalar loads (the simple kernel
+// from its argument segment, the persistent kernel from the image descriptor
+// through the constant address space).
 // The L-BFGS history is kept newest-last and shifted on each update so every
 // register index is static.
 #include "ub_kernels.hpp"

@@ -233,6 +233,15 @@ def config3(size=(4096, 4096), seed=0x5EED0003, image_seed=None):
     return Workload("c3 %dx%d L=7" % (w, h), rgb, layers, 32, 1e-3)
 
 
+def config4(size=(4096, 4096), image_seed=None):
+    """The warm-start configuration: config 2's layer set and generator (5
+    layers, r = 16, eps = 1e-4) on a fresh seeded image, 4096 x 4096 by
+    default; training images pass their own image_seed."""
+    wl = config2(size, image_seed=0x5EED0004 if image_seed is None else image_seed)
+    w, h = size
+    return Workload("c4 %dx%d L=5" % (w, h), wl.rgb, wl.layers, wl.radius, wl.eps)
+
+
 C5_MODES = [NORMAL, MULTIPLY, SCREEN, OVERLAY, COLOR_DODGE, COLOR_BURN, HARD_LIGHT, SOFT_LIGHT,
             DARKEN, LIGHTEN]
 
@@ -259,4 +268,6 @@ def config(k, size=None, image_seed=None):
         return config2(size or (1024, 1024), image_seed=image_seed)
     if k == 3:
         return config3(size or (4096, 4096), image_seed=image_seed)
+    if k == 4:
+        return config4(size or (4096, 4096), image_seed=image_seed)
     raise ValueError("config %r" % k)
