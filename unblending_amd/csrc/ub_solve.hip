@@ -540,8 +540,12 @@ struct WaveQueue {
     bool pending, exhausted;
 };
 
+// A lane's work counts (32 bits: a lane solves far fewer than 2^32 /
+// 1000 pixels of a launch) and the wave's trips around the evaluation loop
+// (wave-uniform).
 struct LaneCounters {
-    unsigned long long evals, iters, outer, pixels, slots;
+    unsigned evals, iters, outer, pixels;
+    unsigned long long trips;
 };
 
 // Claims the next non-empty chunk of the queue: its image and item range.
@@ -734,7 +738,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         }
 
         // ---- one evaluation per lane at the trial point S ----
-        ++lc.slots;
+        ++lc.trips;
         double Ct[PS::NCC];
         const double ft = objective<L, REFINE>(S, lay, pc.tgt, pc.ta, lam, rho, Y, Ct);
 
@@ -888,7 +892,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         const bool run_outer = __popcll(outer_lanes) >= kOuterBatch ||
                                __ballot(phase == kPhStart || phase == kPhTrial) == 0;
         if (run_outer && phase == kPhOuter) {
-            lc.evals += (unsigned long long)nev;
+            lc.evals += (unsigned)nev;
             lc.outer += 1;
             // the constraint vector at the current point: the composite part
             // kept from its evaluation, the refinement's alpha part alpha_i - t_i
@@ -964,19 +968,19 @@ __global__ void __launch_bounds__(64) solve_batch_kernel(BatchArgs args)
     }
     // every image of the launch has L layers (the host groups them so)
     if (q.img >= 0) run_images<L, REFINE>(args, smem, q, lc);
+    unsigned long long evals = lc.evals, iters = lc.iters, outer = lc.outer, pixels = lc.pixels;
     for (int off = 32; off > 0; off >>= 1) {
-        lc.evals += __shfl_xor(lc.evals, off);
-        lc.iters += __shfl_xor(lc.iters, off);
-        lc.outer += __shfl_xor(lc.outer, off);
-        lc.pixels += __shfl_xor(lc.pixels, off);
-        lc.slots += __shfl_xor(lc.slots, off);
+        evals += __shfl_xor(evals, off);
+        iters += __shfl_xor(iters, off);
+        outer += __shfl_xor(outer, off);
+        pixels += __shfl_xor(pixels, off);
     }
     if (threadIdx.x == 0) {
-        atomicAdd(&args.counters[kEvals], lc.evals);
-        atomicAdd(&args.counters[kIterations], lc.iters);
-        atomicAdd(&args.counters[kOuter], lc.outer);
-        atomicAdd(&args.counters[kPixels], lc.pixels);
-        atomicAdd(&args.counters[kSlots], lc.slots);
+        atomicAdd(&args.counters[kEvals], evals);
+        atomicAdd(&args.counters[kIterations], iters);
+        atomicAdd(&args.counters[kOuter], outer);
+        atomicAdd(&args.counters[kPixels], pixels);
+        atomicAdd(&args.counters[kSlots], 64ull * lc.trips);
     }
 }
 
