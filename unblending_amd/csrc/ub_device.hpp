@@ -307,12 +307,16 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     for (int j = 0; j < NC; ++j) squared += C[j] * C[j];
     const double f = energy + (-lagrange) + 0.5 * rho * squared;
 
-    double w[NC];
+    // rho C - lambda: the composite rows here; the refinement's alpha rows
+    // (rho (alpha_i - t_i) - lambda_{3+i}, the same expression) where the
+    // reverse sweep uses them, so that they are not live across it
+    constexpr int NW = REFINE ? 3 : NC;
+    double w[NW];
 #pragma unroll
-    for (int j = 0; j < NC; ++j) w[j] = rho * C[j] - lam[j];
+    for (int j = 0; j < NW; ++j) w[j] = rho * C[j] - lam[j];
 
     double aB[3] = {w[0], w[1], w[2]};
-    double aA = REFINE ? 0.0 : w[3 < NC ? 3 : 0];
+    double aA = REFINE ? 0.0 : w[3 < NW ? 3 : 0];
 #pragma unroll
     for (int k = L - 1; k >= 1; --k) {
         const DevLayer& l = layer_at(lay, k);
@@ -347,7 +351,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
             nB[c] = dB_dcd * aB[c];
         }
         double ga = adj_as + pA_s * aA;
-        if constexpr (REFINE) ga = ga + w[3 + k];
+        if constexpr (REFINE) ga = ga + (rho * (alphas[k] - ta[k]) - lam[3 + k]);
         grad[k] = (REFINE ? grad[k] : 0.01) + ga;
         aA = adj_ad + pA_d * aA;
         aB[0] = nB[0];
@@ -355,7 +359,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         aB[2] = nB[2];
     }
     double ga0 = aA;
-    if constexpr (REFINE) ga0 = ga0 + w[3];
+    if constexpr (REFINE) ga0 = ga0 + (rho * (alphas[0] - ta[0]) - lam[3]);
     grad[0] = (REFINE ? grad[0] : 0.01) + ga0;
     if constexpr (REFINE) {
         const DevLayer& l0 = layer_at(lay, 0);
