@@ -714,9 +714,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 xo[i * 64] = x0[i];
-                double lo, hi;
-                bounds(i, lo, hi);
-                S[i] = clampd(x0[i], lo, hi);
+                xc[i] = x0[i];
             }
             phase = kPhStart;
         } else if (phase == kPhNeedPixel && (q.pending || q.exhausted)) {
@@ -735,6 +733,21 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             q.pending = false;
             phase = kPhNeedPixel;
             continue;
+        }
+
+        // ---- the trial point of every lane, projected on the bounds: the
+        // start point (a new pixel, an outer restart) or the current point
+        // moved t along the direction (a new direction, a backtrack).  One
+        // pass for all lanes, rather than one in each branch that sets a
+        // trial point; lanes without a pixel compute a value nobody reads ----
+        {
+            const bool from_start = phase == kPhStart;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double lo, hi;
+                bounds(i, lo, hi);
+                S[i] = clampd(from_start ? xc[i] : xc[i] + t * dv[i], lo, hi);
+            }
         }
 
         // ---- one evaluation per lane at the trial point S ----
@@ -805,12 +818,6 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 tn = (tn > hi) ? hi : tn;
                 t = tn;
                 ++ls;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    double blo, bhi;
-                    bounds(i, blo, bhi);
-                    S[i] = clampd(xc[i] + t * dv[i], blo, bhi);
-                }
             }
         }
 
@@ -877,12 +884,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 }
                 ls = 0;
 #pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    dv[i] = q[i];
-                    double lo, hi;
-                    bounds(i, lo, hi);
-                    S[i] = clampd(xc[i] + t * q[i], lo, hi);
-                }
+                for (int i = 0; i < N; ++i) dv[i] = q[i];
                 phase = kPhTrial;
             }
         }
@@ -940,12 +942,6 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 ++ocount;
                 count = 0;
                 gamma = 1.0;
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    double lo, hi;
-                    bounds(i, lo, hi);
-                    S[i] = clampd(xc[i], lo, hi);
-                }
                 phase = kPhStart;
             }
         }
