@@ -228,14 +228,13 @@ struct Shape {
 
 // Augmented-Lagrangian objective and its gradient at x.
 // x = [alpha_0..alpha_{L-1}, c_0 rgb .. c_{L-1} rgb].  The forward sweep
-// composites bottom to top and accumulates the energy and its colour
-// gradient; the reverse sweep applies the transposed 4x4 blocks of
-// equations.cpp:222-282 (one reciprocal of A per layer) to rho C - lambda.
-// tgt, ta and lam are indexable (plain arrays, or per-lane LDS vectors); in
-// refinement the alpha slots of grad hold each layer's distance between the
-// two sweeps, which keeps no separate array live; in refinement the colour
-// slots are written in the reverse sweep only (energy and constraint parts
-// together), in the first pass the forward sweep writes the energy part.  Cc receives the
+// composites bottom to top and accumulates the energy; the reverse sweep
+// applies the transposed 4x4 blocks of equations.cpp:222-282 (one reciprocal
+// of A per layer) to rho C - lambda and writes the colour slots of grad
+// (energy and constraint parts together).  tgt, ta and lam are indexable
+// (plain arrays, or per-lane LDS vectors); in refinement the alpha slots of
+// grad hold each layer's distance between the two sweeps, which keeps no
+// separate array live.  Cc receives the
 // constraints that depend on the composite: the colour residual (3) and, in
 // the first pass, the alpha residual (the refinement's alpha constraints
 // alpha_i - t_i need no composite).
@@ -278,12 +277,6 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         const double dist = distance(ck[0], ck[1], ck[2], l);
         if constexpr (REFINE) grad[k] = dist;
         energy += alphas[k] * dist;
-        if constexpr (!REFINE) {
-            double g[3];
-            distance_gradient(ck[0], ck[1], ck[2], l, g);
-#pragma unroll
-            for (int j = 0; j < 3; ++j) grad[L + 3 * k + j] = alphas[k] * g[j];
-        }
     }
 
 #pragma unroll
@@ -329,12 +322,11 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         const double pA_d = l.X * as - l.Y * as + l.Z * (1.0 - as);
         double D[3], gs[3], gd[3];
         blend3_all(mode, cs, cd, D, gs, gd);
-        // refinement: the colour part of dE, alpha_k 2 P (c_k - mu_k), formed
-        // here rather than in the forward sweep, so that it is not live
-        // across both sweeps (same value; the refinement kernel runs 8 %
-        // faster this way, the first-pass kernel 1.5 % slower, measured)
-        double gE[3] = {0.0, 0.0, 0.0};
-        if constexpr (REFINE) distance_gradient(cs[0], cs[1], cs[2], l, gE);
+        // the colour part of dE, alpha_k 2 P (c_k - mu_k), formed here rather
+        // than in the forward sweep, so that it is not live across both
+        // sweeps (same value, fewer spills)
+        double gE[3];
+        distance_gradient(cs[0], cs[1], cs[2], l, gE);
         double adj_as = 0.0, adj_ad = 0.0;
         double nB[3];
 #pragma unroll
@@ -344,8 +336,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
             const double dB_das = ((D[c] * ad + l.Y * (1.0 - ad) * cs[c] - l.Z * ad * cd[c]) - B * pA_s) * invA;
             const double dB_dcd = (as * ad * gd[c] + l.Z * (1.0 - as) * ad) * invA;
             const double dB_dad = ((D[c] * as - l.Y * as * cs[c] + l.Z * (1.0 - as) * cd[c]) - B * pA_d) * invA;
-            if constexpr (REFINE) grad[L + 3 * k + c] = alphas[k] * gE[c] + dB_dcs * aB[c];
-            else grad[L + 3 * k + c] = grad[L + 3 * k + c] + dB_dcs * aB[c];
+            grad[L + 3 * k + c] = alphas[k] * gE[c] + dB_dcs * aB[c];
             adj_as += dB_das * aB[c];
             adj_ad += dB_dad * aB[c];
             nB[c] = dB_dcd * aB[c];
@@ -361,15 +352,12 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     double ga0 = aA;
     if constexpr (REFINE) ga0 = ga0 + (rho * (alphas[0] - ta[0]) - lam[3]);
     grad[0] = (REFINE ? grad[0] : 0.01) + ga0;
-    if constexpr (REFINE) {
+    {
         const DevLayer& l0 = layer_at(lay, 0);
         double gE[3];
         distance_gradient(colors[0], colors[1], colors[2], l0, gE);
 #pragma unroll
         for (int c = 0; c < 3; ++c) grad[L + c] = alphas[0] * gE[c] + aB[c];
-    } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) grad[L + c] = grad[L + c] + aB[c];
     }
     return f;
 }
