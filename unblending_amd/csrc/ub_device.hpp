@@ -210,12 +210,24 @@ __device__ __forceinline__ int uniform_mode(int mode)
 }
 
 // The layer table is uniform and read with scalar loads.  Indexing it through
-// an opaque zero pins each layer's loads next to their use: otherwise the
-// scheduler hoists all layers' parameters together and spills the SGPRs.
+// an opaque zero keeps the loads from being merged with, or hoisted to, the
+// loads of another sweep; otherwise the scheduler loads all layers'
+// parameters together and spills the SGPRs.
 __device__ __forceinline__ const DevLayer& layer_at(const DevLayer* lay, int k)
 {
     int zero;
     asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+    return lay[k + zero];
+}
+
+// The same, with the offset also depending on dep: the loads cannot be issued
+// before dep exists.  The sweeps pass a value computed early in the step
+// before the layer's, so each layer's parameters are loaded one step ahead
+// (the scalar-load latency is covered) and not held across earlier steps.
+__device__ __forceinline__ const DevLayer& layer_after(const DevLayer* lay, int k, double dep)
+{
+    int zero;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(zero) : "v"(dep));
     return lay[k + zero];
 }
 
@@ -252,7 +264,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     double energy = 0.0;
 #pragma unroll
     for (int k = 0; k < L; ++k) {
-        const DevLayer& l = layer_at(lay, k);
+        const DevLayer& l = k < 2 ? layer_at(lay, k) : layer_after(lay, k, pre[k - 2][3]);
         const double* ck = colors + 3 * k;
         if (k == 0) {
             pre[0][0] = ck[0];
@@ -310,14 +322,16 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
 
     double aB[3] = {w[0], w[1], w[2]};
     double aA = REFINE ? 0.0 : w[3 < NW ? 3 : 0];
+    double dep = w[0];
 #pragma unroll
     for (int k = L - 1; k >= 1; --k) {
-        const DevLayer& l = layer_at(lay, k);
+        const DevLayer& l = layer_after(lay, k, dep);
         const int mode = uniform_mode(l.mode);
         const double* cs = colors + 3 * k;
         const double* cd = pre[k - 1];
         const double as = alphas[k], ad = pre[k - 1][3];
         const double invA = 1.0 / pre[k][3];
+        dep = invA;
         const double pA_s = l.X * ad + l.Y * (1.0 - ad) - l.Z * ad;
         const double pA_d = l.X * as - l.Y * as + l.Z * (1.0 - as);
         double D[3], gs[3], gd[3];
@@ -353,7 +367,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     if constexpr (REFINE) ga0 = ga0 + (rho * (alphas[0] - ta[0]) - lam[3]);
     grad[0] = (REFINE ? grad[0] : 0.01) + ga0;
     {
-        const DevLayer& l0 = layer_at(lay, 0);
+        const DevLayer& l0 = layer_after(lay, 0, dep);
         double gE[3];
         distance_gradient(colors[0], colors[1], colors[2], l0, gE);
 #pragma unroll
