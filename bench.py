@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of autotuned ones")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus)
+        print("bench.py runs on one GPU: the library has no multi-GPU path (--gpus %d requested)" % args.gpus)
         sys.exit(2)
 
     import torch

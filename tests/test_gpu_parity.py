@@ -3,7 +3,7 @@
 The device solve runs the same algorithm in the same FP64 operation order as
 the oracle, and the guided filter takes its box means from exact integer sums
 (the oracle's exact-sum filter), so both passes agree with the oracle to the
-last bit; that is asserted, together with the bar the path is judged by:
+last bit; that is asserted, together with the tolerances of the path:
 per-pixel alphas and colours within 1e-3 max abs, recomposite RMS difference
 at most 1e-4, guided filter within 1e-6 of the FP64 brute-force filter.
 """
