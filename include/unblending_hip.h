@@ -141,7 +141,9 @@ UNBLENDING_API int32_t unblending_hip_mlp_train(
     const double* targets, int32_t layers, int32_t steps, int32_t steps_done, int32_t batch,
     float learning_rate, uint64_t seed, const int64_t* fixed_samples, int32_t update,
     void* workspace, size_t workspace_size, void* stream, float* losses, float* gradients);
-/* Predicted first pass for one image, packed [layers][height*width][RGBA]. */
+/* Predicted first pass for one image, packed [layers][height*width][RGBA].
+ * One fused kernel: no workspace is needed (the size is 0; workspace may be
+ * NULL), the arguments are kept for callers that pass one. */
 UNBLENDING_API size_t unblending_hip_mlp_predict_workspace_size(void);
 UNBLENDING_API int32_t unblending_hip_mlp_predict(
     const void* model, const double* rgb, int32_t width, int32_t height, int32_t layers,

@@ -192,7 +192,8 @@ def test_device_entry_points_validate_workspace():
     L = layers(3)
     assert lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 0) >= 32
     assert lib.unblending_hip_workspace_size(4, 3, 3, 1, 1, 0) > lib.unblending_hip_workspace_size(4, 3, 3, 1, 0, 0)
-    assert lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 1) > lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 0)
+    # the warm start's prediction is one fused kernel: no extra workspace
+    assert lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 1) == lib.unblending_hip_workspace_size(4, 3, 3, 0, 0, 0)
     assert lib.unblending_hip_workspace_size(0, 3, 3, 1, 0, 0) == 0
     opt = U.options(U.BACKEND_HIP)
     # null rgb, then a too-small workspace: rejected before any device work
@@ -210,12 +211,12 @@ def test_warm_start_model_interface_checks():
     assert lib.unblending_hip_mlp_train_workspace_size(0) == 0
     assert lib.unblending_hip_mlp_train_workspace_size(100) == 0
     assert lib.unblending_hip_mlp_train_workspace_size(128) < lib.unblending_hip_mlp_train_workspace_size(256)
-    assert lib.unblending_hip_mlp_predict_workspace_size() > 0
+    assert lib.unblending_hip_mlp_predict_workspace_size() == 0  # one fused kernel
     buf = ctypes.c_void_p(16)  # never dereferenced: every call below fails its argument checks first
     # more than 8 layers, bad batch, missing buffers
     assert lib.unblending_hip_mlp_predict(buf, buf, 4, 4, 9, buf, buf, 1 << 30, None) == U.INVALID_ARGUMENT
     assert lib.unblending_hip_mlp_predict(None, buf, 4, 4, 3, buf, buf, 1 << 30, None) == U.INVALID_ARGUMENT
-    assert lib.unblending_hip_mlp_predict(buf, buf, 4, 4, 3, buf, buf, 16, None) == U.BUFFER_TOO_SMALL
+    assert lib.unblending_hip_mlp_predict(buf, buf, 0, 4, 3, buf, None, 0, None) == U.INVALID_ARGUMENT
     assert lib.unblending_hip_mlp_train(buf, buf, 1, 4, 4, buf, 3, 1, 0, 100, 1e-3, 0, None, 1, buf, 1 << 30,
                                         None, None, None) == U.INVALID_ARGUMENT
     assert lib.unblending_hip_mlp_train(buf, buf, 1, 4, 4, buf, 3, 1, 0, 128, 1e-3, 0, None, 1, buf, 16,

@@ -660,7 +660,7 @@ UNBLENDING_API int32_t unblending_hip_mlp_train(void* model, const double* image
     return UNBLENDING_OK;
 }
 
-UNBLENDING_API size_t unblending_hip_mlp_predict_workspace_size(void) { return mlp_predict_bytes(); }
+UNBLENDING_API size_t unblending_hip_mlp_predict_workspace_size(void) { return 0; }
 
 UNBLENDING_API int32_t unblending_hip_mlp_predict(const void* model, const double* rgb, int32_t width,
                                                   int32_t height, int32_t layers, double* output, void* workspace,
@@ -669,9 +669,10 @@ UNBLENDING_API int32_t unblending_hip_mlp_predict(const void* model, const doubl
     g_last_error.clear();
     if (!model || !rgb || !output || width <= 0 || height <= 0 || layers <= 0 || 4 * layers > mlp::kOutPad)
         return UNBLENDING_INVALID_ARGUMENT;
-    if (!workspace || workspace_size < mlp_predict_bytes()) return UNBLENDING_BUFFER_TOO_SMALL;
+    (void)workspace;  // the fused prediction kernel needs none
+    (void)workspace_size;
     const hipError_t e = mlp_predict(mlp::model_view(const_cast<void*>(model)), rgb, width, height, 4 * layers,
-                                     output, mlp_predict_view(workspace), static_cast<hipStream_t>(stream));
+                                     output, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) {
         g_last_error = hipGetErrorString(e);
         return UNBLENDING_DEVICE_ERROR;

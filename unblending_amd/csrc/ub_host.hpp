@@ -52,9 +52,6 @@ struct DeviceJob {
 size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool warm);
 size_t mlp_train_bytes(int batch);
 mlp::TrainBuffers mlp_train_view(void* ws, int batch);
-// prediction buffers for an image of n pixels (the default: the largest chunk)
-size_t mlp_predict_bytes(size_t n = size_t(1) << 40);
-mlp::PredictBuffers mlp_predict_view(void* ws, size_t n = size_t(1) << 40);
 size_t filter_workspace_bytes(size_t n, int channels);
 void run_device(const DeviceJob& job);
 void run_filter(const double* rgb, int w, int h, const Channel* in, double* const* q, int channels, int radius,

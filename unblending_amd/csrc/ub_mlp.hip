@@ -25,20 +25,34 @@ namespace mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// The logistic function 1 / (1 + e^-x) from the hardware base-2 exponential
+// and reciprocal (about 1 ulp each; the products round to BF16 anyway):
+// straight-line code, saturating to 0 and 1 for large |x|.
+constexpr float kLog2e = 1.4426950408889634f;
+
+__device__ __forceinline__ float logistic(float x)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-kLog2e * x));
+}
+
+// GELU, tanh form: 0.5 z (1 + tanh u) with u = c0 (z + c1 z^3), which is
+// z * logistic(2u) since 1 + tanh u = 2 logistic(2u).
 __device__ __forceinline__ float gelu(float z)
 {
     const float u = kGeluC0 * (z + kGeluC1 * z * z * z);
-    return 0.5f * z * (1.0f + tanhf(u));
+    return z * logistic(2.0f * u);
 }
 
+// d/dz of the same: 0.5 (1 + t) + 0.5 z (1 - t^2) u'(z) with t = tanh u,
+// 0.5 (1 + t) = s = logistic(2u) and 1 - t^2 = 4 s (1 - s).
 __device__ __forceinline__ float gelu_grad(float z)
 {
     const float u = kGeluC0 * (z + kGeluC1 * z * z * z);
-    const float t = tanhf(u);
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC0 * (1.0f + 3.0f * kGeluC1 * z * z);
+    const float s = logistic(2.0f * u);
+    return s + 2.0f * z * s * (1.0f - s) * kGeluC0 * (1.0f + 3.0f * kGeluC1 * z * z);
 }
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+__device__ __forceinline__ float sigmoid(float z) { return logistic(z); }
 
 // Row loader for a dense BF16 operand.
 struct DenseRows {
@@ -304,6 +318,85 @@ __global__ void zero_f32_kernel(float* p, int count)
     if (i < count) p[i] = 0.0f;
 }
 
+// Inference, all four products of a tile of kPredictRows pixels in one
+// workgroup: wave v owns tile rows 32 v .. 32 v + 31 and computes every output
+// column of its rows, keeping its hidden activations in LDS (BF16, row-major,
+// overwritten layer by layer), so waves share no data and nothing but the
+// image and the prediction crosses HBM.  The reduction order of every product
+// (k ascending, one 32x32x16 MFMA per step) and the epilogue expressions are
+// those of gemm_kernel with EpiHidden and EpiOutputPredict: the prediction is
+// the layer-by-layer one, bit for bit.
+constexpr int kPredictRows = 128;
+constexpr int kActStride = kHidden + 8;  // BF16 per LDS row; the pad spreads rows over banks
+
+__global__ void __launch_bounds__(256) predict_kernel(Model md, const double* rgb, int w, int h, long long n,
+                                                      int outputs, double* out)
+{
+    __shared__ __bf16 act[kPredictRows * kActStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int half = lane >> 5, r32 = lane & 31;
+    const long long base = (long long)blockIdx.x * kPredictRows;
+    const long long rows = (n - base < kPredictRows) ? n - base : kPredictRows;
+    const int m0 = wave * 32;
+    __bf16* wa = act + m0 * kActStride;  // this wave's rows
+
+    // first layer: the 3x3 patches (rows past the image repeat its last pixel)
+    const PatchRows patches{rgb, nullptr, rows, w, h, base};
+    f32x16 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[b] = f32x16{};
+#pragma unroll
+    for (int k0 = 0; k0 < kInPad; k0 += 16) {
+        const bf16x8 av = patches.load(m0 + r32, k0 + 8 * half);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(md.wt[0] + (b * 32 + r32) * kInPad + k0 + 8 * half);
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[b], 0, 0, 0);
+        }
+    }
+    for (int l = 0;; ++l) {
+        // epilogue of hidden layer l: GELU(z + b) into this wave's rows
+        const float* bias = md.w + kOffB[l];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 8 * (r >> 2) + 4 * half + (r & 3), col = b * 32 + r32;
+                wa[row * kActStride + col] = (__bf16)gelu(acc[b][r] + bias[col]);
+            }
+        __syncthreads();
+        if (l == 2) break;
+        // hidden layer l + 1
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = f32x16{};
+        for (int k0 = 0; k0 < kHidden; k0 += 16) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(wa + r32 * kActStride + k0 + 8 * half);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bf16x8 bv =
+                    *reinterpret_cast<const bf16x8*>(md.wt[l + 1] + (b * 32 + r32) * kHidden + k0 + 8 * half);
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[b], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every read of layer l's activations precedes their overwrite
+    }
+    // output layer (32 columns) and sigmoid, written as a packed first pass
+    f32x16 y = {};
+    for (int k0 = 0; k0 < kHidden; k0 += 16) {
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(wa + r32 * kActStride + k0 + 8 * half);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(md.wt[3] + r32 * kHidden + k0 + 8 * half);
+        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, y, 0, 0, 0);
+    }
+    const float* bias = md.w + kOffB[3];
+    const int j = r32;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const long long m = m0 + 8 * (r >> 2) + 4 * half + (r & 3);
+        const long long p = base + m;
+        if (j < outputs && m < rows) out[((long long)(j >> 2) * n + p) * 4 + (j & 3)] = (double)sigmoid(y[r] + bias[j]);
+    }
+}
+
 template <class ALoad, class Epi>
 hipError_t gemm(const ALoad& a, const __bf16* bt, long long M, int N, int K, const Epi& epi, hipStream_t s)
 {
@@ -398,26 +491,13 @@ hipError_t mlp_step(const Model& md, const TrainBatch& tb, const TrainBuffers& b
     return hipGetLastError();
 }
 
-hipError_t mlp_predict(const Model& md, const double* rgb, int w, int h, int outputs, double* out,
-                       const PredictBuffers& buf, hipStream_t s)
+hipError_t mlp_predict(const Model& md, const double* rgb, int w, int h, int outputs, double* out, hipStream_t s)
 {
     const long long n = (long long)w * h;
-    hipError_t e;
-    for (long long base = 0; base < n; base += buf.chunk) {
-        const long long rows = (n - base < buf.chunk) ? n - base : buf.chunk;
-        const long long M = (rows + 31) / 32 * 32;
-        // rows of this chunk: pixels base .. base + rows - 1 of the image
-        const PatchRows po{rgb, nullptr, rows, w, h, base};
-        if ((e = gemm(po, md.wt[0], M, kHidden, kInPad,
-                      EpiHidden{md.w + kOffB[0], nullptr, buf.h[0], nullptr, M, kHidden}, s)) != hipSuccess) return e;
-        for (int l = 1; l < 3; ++l)
-            if ((e = gemm(DenseRows{buf.h[l - 1], kHidden}, md.wt[l], M, kHidden, kHidden,
-                          EpiHidden{md.w + kOffB[l], nullptr, buf.h[l], nullptr, M, kHidden}, s)) != hipSuccess)
-                return e;
-        if ((e = gemm(DenseRows{buf.h[2], kHidden}, md.wt[3], M, kOutPad, kHidden,
-                      EpiOutputPredict{md.w + kOffB[3], out, n, base, outputs}, s)) != hipSuccess) return e;
-    }
-    return hipSuccess;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(predict_kernel, dim3((unsigned)((n + kPredictRows - 1) / kPredictRows)), dim3(256), 0, s, md,
+                       rgb, w, h, n, outputs, out);
+    return hipGetLastError();
 }
 
 }  // namespace ubhip

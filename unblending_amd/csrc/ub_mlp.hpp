@@ -107,11 +107,6 @@ constexpr int kGradSplits = 32;  // workgroups along the batch of a weight-gradi
 
 constexpr int kLossSlots = 1024;
 
-struct PredictBuffers {
-    long long chunk;
-    __bf16* h[3];
-};
-
 }  // namespace mlp
 
 hipError_t mlp_refresh(const mlp::Model& md, hipStream_t s);
@@ -119,7 +114,9 @@ hipError_t mlp_refresh(const mlp::Model& md, hipStream_t s);
 // buf.grad, the batch loss into *loss (device), and an Adam update if asked.
 hipError_t mlp_step(const mlp::Model& md, const mlp::TrainBatch& tb, const mlp::TrainBuffers& buf, int step,
                     bool update, float lr, float* loss, hipStream_t s);
+// The prediction for one image, packed [outputs / 4][w*h][RGBA]; one fused
+// kernel, no workspace.
 hipError_t mlp_predict(const mlp::Model& md, const double* rgb, int w, int h, int outputs, double* out,
-                       const mlp::PredictBuffers& buf, hipStream_t s);
+                       hipStream_t s);
 
 }  // namespace ubhip

@@ -131,7 +131,6 @@ size_t filter_workspace_bytes(size_t n, int channels)
 }
 
 namespace {
-constexpr long long kPredictChunk = 1 << 18;  // pixels per prediction chunk
 
 // Carves `bytes` (256-aligned) from a workspace cursor.
 template <class T>
@@ -142,22 +141,6 @@ T* carve(char*& cur, size_t count)
     return p;
 }
 }  // namespace
-
-namespace {
-// rows of one prediction chunk for an n-pixel image (a multiple of 32)
-long long predict_chunk(size_t n) { return std::min<long long>(kPredictChunk, (long long)((n + 31) / 32 * 32)); }
-}  // namespace
-
-size_t mlp_predict_bytes(size_t n) { return 3 * align256(size_t(predict_chunk(n)) * mlp::kHidden * 2); }
-
-mlp::PredictBuffers mlp_predict_view(void* ws, size_t n)
-{
-    mlp::PredictBuffers b;
-    char* cur = static_cast<char*>(ws);
-    b.chunk = predict_chunk(n);
-    for (int l = 0; l < 3; ++l) b.h[l] = carve<__bf16>(cur, size_t(b.chunk) * mlp::kHidden);
-    return b;
-}
 
 mlp::TrainBuffers mlp_train_view(void* ws, int batch)
 {
@@ -193,8 +176,8 @@ size_t mlp_train_bytes(int batch)
 
 size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool warm)
 {
+    (void)warm;  // the warm start's prediction needs no workspace
     size_t bytes = kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
-    if (warm && !refine) bytes += mlp_predict_bytes(n);
     if (refine) {
         const int C = filter_channels(layers, smooth);
         bytes += filter_workspace_bytes(n, C);
@@ -301,14 +284,12 @@ void run_device(const DeviceJob& job)
     if (warm) {
         if (!job.params.mlp) throw std::invalid_argument("A warm start needs a model");
         if (4 * L > size_t(mlp::kOutPad)) throw ExecutionError("The warm-start model predicts at most 8 layers");
-        char* wcur = ws + kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
-        const mlp::PredictBuffers pb = mlp_predict_view(wcur, n);
         const mlp::Model md = mlp::model_view(const_cast<void*>(job.params.mlp));
         Events ep(timing);
         ep.start(job.stream);
         // the prediction is written into the output buffer, which the solve
         // reads pixel by pixel before writing the same pixel's solution
-        check(mlp_predict(md, job.rgb, job.w, job.h, int(4 * L), job.out, pb, job.stream), "warm-start prediction");
+        check(mlp_predict(md, job.rgb, job.w, job.h, int(4 * L), job.out, job.stream), "warm-start prediction");
         ep.stop(job.stream);
         predict_ms = ep.ms();
         args.warm = job.out;
