@@ -454,13 +454,15 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // oracle.
 //
 // Registers hold the current point and the (s, y) pair; the pair's registers
-// hold the trial point and trial gradient between a search direction and its
-// acceptance (the pair is no longer needed once the direction is computed).
-// LDS holds, per lane, the gradient, the search direction, the multipliers
-// the constraints that depend on the composite at the current point (kept
-// from that point's evaluation) and, in refinement where the layer count
-// leaves room, the pixel's target colour and target alphas, so that an
-// evaluation's live set fits the 256 architectural registers of a lane.
+// hold the trial point and trial gradient from the projection that starts a
+// trip until the trial is accepted (the pair is no longer needed once the
+// direction is computed).  LDS holds, per lane, the gradient, the search
+// direction, the multipliers, the constraints that depend on the composite
+// at the current point (kept from that point's evaluation) and, in
+// refinement where the layer count leaves room, the pixel's target colour and
+// target alphas, so that an evaluation's live set stays near the 256
+// architectural registers of a lane (the excess goes to accumulation
+// registers, not to scratch).
 // Only the outer iterate, read and written once per outer iteration, has a
 // per-lane global slot.
 // ---------------------------------------------------------------------------
