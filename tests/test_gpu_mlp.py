@@ -53,7 +53,7 @@ def test_forward_matches_oracle():
     params = _perturbed_params(L, 1)
     model = mlp.WarmStartModel(L, params=params)
     rng = np.random.default_rng(0)
-    rgb = rng.uniform(0, 1, (23, 37, 3))  # 851 pixels: a partial tile of 32 rows
+    rgb = rng.uniform(0, 1, (23, 37, 3))  # 851 pixels: the last 128-pixel tile is partial
     print("case forward 37x23 L=5", flush=True)
     y = model.predict(_dev(rgb)).cpu().numpy()
     ref = M.predict(params, rgb, L)
@@ -63,8 +63,9 @@ def test_forward_matches_oracle():
 
 
 def test_forward_across_prediction_chunks():
-    """An image larger than one prediction chunk (262144 pixels): pixels on
-    both sides of the chunk boundary and at the end."""
+    """A large image (280000 pixels, 2188 tiles of 128, the last one partial):
+    pixels at the start, around pixel 262144 and at the end, whose 3x3
+    patches cross image rows and edges."""
     from unblending_amd import mlp
     L = 3
     params = _perturbed_params(L, 2)
