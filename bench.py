@@ -234,6 +234,7 @@ def main():
     ap.add_argument("--train-size", type=int, default=512)
     ap.add_argument("--train-images", type=int, default=4)
     ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the reference point "
                          "for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -346,7 +347,7 @@ def main():
         label_s = time.perf_counter() - t0
         model = mlp.WarmStartModel(L, device=dev, seed=0)
         t0 = time.perf_counter()
-        losses = model.fit(images, targets, steps=args.train_steps, batch=4096, lr=args.train_lr,
+        losses = model.fit(images, targets, steps=args.train_steps, batch=args.train_batch, lr=args.train_lr,
                            final_lr=0.05 * args.train_lr, chunk=1000)
         torch.cuda.synchronize()
         train_s = time.perf_counter() - t0
@@ -373,7 +374,7 @@ def main():
         warm = {"ms_per_step": ms_warm, "s1": w1, "s2": w2, "train_s": train_s, "label_s": label_s,
                 "quality": {"rows": band, "plain": qp, "warm": qw}, "recomposite": warm_recon,
                 "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-100:])),
-                "train_tflops": mlp_train_flops(4096) / per_train / 1e12 if per_train > 0 else 0.0,
+                "train_tflops": mlp_train_flops(args.train_batch) / per_train / 1e12 if per_train > 0 else 0.0,
                 "first_max": float(d1.max()), "first_mean": float(d1.mean()),
                 "first_frac_gt_1e-3": float((d1 > 1e-3).double().mean()),
                 "refined_max": float(d2.max()), "refined_mean": float(d2.mean()),
@@ -449,7 +450,7 @@ def main():
             "first_pass_quality": warm["quality"],
             "recomposite": warm["recomposite"],
             "training": {"images": args.train_images, "size": args.train_size, "steps": args.train_steps,
-                         "batch": 4096, "lr": args.train_lr, "label_s": round(warm["label_s"], 3),
+                         "batch": args.train_batch, "lr": args.train_lr, "label_s": round(warm["label_s"], 3),
                          "train_s": round(warm["train_s"], 3), "loss_first": warm["loss_first"],
                          "loss_last": warm["loss_last"], "mfma_tflops": round(warm["train_tflops"], 3),
                          "mfma_frac_bf16": round(warm["train_tflops"] / BF16_PEAK_TFLOPS, 5)},
