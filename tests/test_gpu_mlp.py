@@ -83,6 +83,28 @@ def test_forward_across_prediction_chunks():
     assert d.max() <= FORWARD_TOL
 
 
+def test_predict_needs_no_workspace():
+    """The C entry point with a null workspace of size 0 (the documented
+    size) gives the prediction bit for bit."""
+    import ctypes
+    import torch
+    import unblending_amd as U
+    from unblending_amd import mlp
+    L = 4
+    model = mlp.WarmStartModel(L, params=_perturbed_params(L, 3))
+    rgb = _dev(np.random.default_rng(3).uniform(0, 1, (19, 31, 3)))
+    print("case predict without workspace 31x19 L=4", flush=True)
+    ref = model.predict(rgb)
+    out = torch.empty_like(ref)
+    lib = U.lib()
+    assert lib.unblending_hip_mlp_predict_workspace_size() == 0
+    st = lib.unblending_hip_mlp_predict(ctypes.c_void_p(model.data_ptr()), ctypes.c_void_p(rgb.data_ptr()), 31, 19,
+                                        L, ctypes.c_void_p(out.data_ptr()), None, 0, None)
+    assert st == U.OK
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+
+
 def _train_problem(L=4, count=2, w=29, h=19, seed=0):
     rng = np.random.default_rng(seed)
     images = rng.uniform(0, 1, (count, h, w, 3))
