@@ -107,3 +107,29 @@ def test_guided_filter_launch_shapes(oracle, block, rows):
         ref = oracle.guided_filter(p, rgb, r, eps, brute=True)
         assert np.abs(q.cpu().numpy() - ref).max() <= FILTER_TOL
         assert np.array_equal(q.cpu().numpy(), oracle.guided_filter_exact(p, rgb, r, eps))
+
+
+def test_refine_first_pass_may_alias_output():
+    """The C ABI's refinement may take its first pass and its output in the
+    same buffer (unblending/include/unblending.h:96-97): the result equals the
+    call with separate buffers, bit for bit."""
+    import ctypes
+    import unblending_amd as U
+    wl = _workload(2, (33, 29))
+    infos = layer_infos(wl.layers)
+    L = len(infos)
+    print("case refine in place c2 33x29 L=%d" % L, flush=True)
+    first = U.compute_color_unmixing(wl.rgb, infos, True)
+    v, keep = U._view(wl.rgb)
+    opt = U.options(U.BACKEND_HIP, True, False)
+    dp = ctypes.POINTER(ctypes.c_double)
+    separate = np.empty_like(first)
+    st = U.lib().unblending_refine(ctypes.byref(v), U.layer_array(infos), L, first.ctypes.data_as(dp), first.size,
+                                   ctypes.byref(opt), separate.ctypes.data_as(dp), separate.size)
+    assert st == U.OK
+    buf = first.copy()
+    st = U.lib().unblending_refine(ctypes.byref(v), U.layer_array(infos), L, buf.ctypes.data_as(dp), buf.size,
+                                   ctypes.byref(opt), buf.ctypes.data_as(dp), buf.size)
+    assert st == U.OK
+    assert np.array_equal(buf, separate)
+    assert not np.array_equal(separate, first)
