@@ -133,3 +133,33 @@ def test_refine_first_pass_may_alias_output():
     assert st == U.OK
     assert np.array_equal(buf, separate)
     assert not np.array_equal(separate, first)
+
+
+def test_concurrent_calls_from_two_threads():
+    """The C ABI is re-entrant (each call has its own stream and buffers): two
+    host threads unmixing different images at once get the results of the
+    same calls made one after the other."""
+    import threading
+    import unblending_amd as U
+    work = [_workload(1, (40, 36)), _workload(2, (28, 44))]
+    print("case two threads c1 40x36 + c2 28x44", flush=True)
+    alone = [U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True) for wl in work]
+    got = [None, None]
+    errors = []
+
+    def run(i):
+        try:
+            for _ in range(3):
+                got[i] = U.compute_color_unmixing(work[i].rgb, layer_infos(work[i].layers), True)
+        except Exception as e:  # reported below, after the join
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+    for i in range(2):
+        assert np.array_equal(got[i], alone[i])
