@@ -599,8 +599,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     // precomputed 64-bit addresses live across the loop
     double* const xo_base = args.lane_state + (long long)blockIdx.x * (4 * kMaxLayers * 64) + lane;
 
-    const SolverParams sp = args.sp;
-    const bool afixed = REFINE && sp.alpha_fixed != 0;
+    const bool afixed = REFINE && args.sp.alpha_fixed != 0;
     bool opaque = false, smooth = false;
 
     // S, Y: the (s, y) pair, and between direction and acceptance the trial
@@ -655,6 +654,9 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         opaque = dd->opaque != 0;
         smooth = REFINE && dd->smooth != 0;
         double* const xo = xo_base + zero;
+        // the solver constants, read where they are used (scalar loads at the
+        // opaque offset) rather than held in scalar registers for the launch
+        const SolverParams& sp = (&args.sp)[zero];
 
         // ---- next pixels for the lanes that finished theirs ----
         // (handed out in batches of at least kFetchBatch lanes while others are
