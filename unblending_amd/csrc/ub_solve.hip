@@ -593,11 +593,6 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     const LaneVec cc{smem + (2 * N + NC) * 64 + lane};
     PC pc(smem + (2 * N + NC + PS::NCC) * 64 + lane);
     double bg[3] = {0.0, 0.0, 0.0};  // background colour (smoothed refinement)
-    // the outer iterate, touched once per outer iteration: a per-lane global
-    // slot, [workgroup][variable][lane]; its address is formed inside the
-    // loop (from the opaque zero below) so that the compiler does not keep N
-    // precomputed 64-bit addresses live across the loop
-    double* const xo_base = args.lane_state + (long long)blockIdx.x * (4 * kMaxLayers * 64) + lane;
 
     const bool afixed = REFINE && args.sp.alpha_fixed != 0;
     bool opaque = false, smooth = false;
@@ -653,7 +648,12 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         const DevLayer* lay = (const DevLayer*)dd->layers;
         opaque = dd->opaque != 0;
         smooth = REFINE && dd->smooth != 0;
-        double* const xo = xo_base + zero;
+        // the outer iterate, touched once per outer iteration: a per-lane
+        // global slot, [workgroup][variable][lane], its address formed here
+        // from the argument read at the opaque offset, so that neither the
+        // slot's base nor N precomputed 64-bit addresses stay live across
+        // the loop
+        double* const xo = (&args.lane_state)[zero] + (long long)blockIdx.x * (4 * kMaxLayers * 64) + lane;
         // the solver constants, read where they are used (scalar loads at the
         // opaque offset) rather than held in scalar registers for the launch
         const SolverParams& sp = (&args.sp)[zero];
@@ -673,7 +673,9 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 if (q.pending || q.exhausted) break;
                 int i2;
                 long long b2, e2;
-                if (!claim_chunk(args, i2, b2, e2)) {
+                // the launch arguments at the opaque offset too: claim_chunk's
+                // reads of them stay here instead of in launch-long registers
+                if (!claim_chunk((&args)[zero], i2, b2, e2)) {
                     q.exhausted = true;
                     break;
                 }
