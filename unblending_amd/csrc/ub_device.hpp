@@ -240,13 +240,15 @@ struct Shape {
 
 // Augmented-Lagrangian objective and its gradient at x.
 // x = [alpha_0..alpha_{L-1}, c_0 rgb .. c_{L-1} rgb].  The forward sweep
-// composites bottom to top and accumulates the energy; the reverse sweep
-// applies the transposed 4x4 blocks of equations.cpp:222-282 (one reciprocal
-// of A per layer) to rho C - lambda and writes the colour slots of grad
-// (energy and constraint parts together).  tgt, ta and lam are indexable
-// (plain arrays, or per-lane LDS vectors); in refinement the alpha slots of
-// grad hold each layer's distance between the two sweeps, which keeps no
-// separate array live.  Cc receives the
+// composites bottom to top (it reads only the comp-op factors and blend mode
+// of each layer); the reverse sweep applies the transposed 4x4 blocks of
+// equations.cpp:222-282 (one reciprocal of A per layer) to rho C - lambda,
+// evaluates each layer's colour model (distance and its gradient) and writes
+// grad (energy and constraint parts together); the energy is then summed
+// bottom to top, in the reference's order.  Evaluating the colour models in
+// the reverse sweep only keeps each layer's mean and precision out of the
+// registers of the forward sweep (same values).  tgt, ta and lam are
+// indexable (plain arrays, or per-lane LDS vectors).  Cc receives the
 // constraints that depend on the composite: the colour residual (3) and, in
 // the first pass, the alpha residual (the refinement's alpha constraints
 // alpha_i - t_i need no composite).
@@ -261,7 +263,6 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
 
     double pre[L][4];
     double C[NC];
-    double energy = 0.0;
 #pragma unroll
     for (int k = 0; k < L; ++k) {
         const DevLayer& l = k < 2 ? layer_at(lay, k) : layer_after(lay, k, pre[k - 2][3]);
@@ -286,9 +287,6 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
             }
             pre[k][3] = a;
         }
-        const double dist = distance(ck[0], ck[1], ck[2], l);
-        if constexpr (REFINE) grad[k] = dist;
-        energy += alphas[k] * dist;
     }
 
 #pragma unroll
@@ -298,10 +296,6 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         for (int i = 0; i < L; ++i) C[3 + i] = alphas[i] - ta[i];
     } else {
         C[3] = pre[L - 1][3] - 1.0;
-        double sum = 0.0;
-#pragma unroll
-        for (int i = 0; i < L; ++i) sum += alphas[i];
-        energy += 0.01 * sum;
     }
 #pragma unroll
     for (int j = 0; j < (REFINE ? 3 : 4); ++j) Cc[j] = C[j];
@@ -310,7 +304,6 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     for (int j = 0; j < NC; ++j) lagrange += lam[j] * C[j];
 #pragma unroll
     for (int j = 0; j < NC; ++j) squared += C[j] * C[j];
-    const double f = energy + (-lagrange) + 0.5 * rho * squared;
 
     // rho C - lambda: the composite rows here; the refinement's alpha rows
     // (rho (alpha_i - t_i) - lambda_{3+i}, the same expression) where the
@@ -323,6 +316,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     double aB[3] = {w[0], w[1], w[2]};
     double aA = REFINE ? 0.0 : w[3 < NW ? 3 : 0];
     double dep = w[0];
+    double dist[L];  // the layers' colour-model distances (the energy's terms)
 #pragma unroll
     for (int k = L - 1; k >= 1; --k) {
         const DevLayer& l = layer_after(lay, k, dep);
@@ -336,11 +330,11 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         const double pA_d = l.X * as - l.Y * as + l.Z * (1.0 - as);
         double D[3], gs[3], gd[3];
         blend3_all(mode, cs, cd, D, gs, gd);
-        // the colour part of dE, alpha_k 2 P (c_k - mu_k), formed here rather
-        // than in the forward sweep, so that it is not live across both
-        // sweeps (same value, fewer spills)
+        // the colour model of layer k: the colour part of dE,
+        // alpha_k 2 P (c_k - mu_k), and the distance for the energy
         double gE[3];
         distance_gradient(cs[0], cs[1], cs[2], l, gE);
+        dist[k] = distance(cs[0], cs[1], cs[2], l);
         double adj_as = 0.0, adj_ad = 0.0;
         double nB[3];
 #pragma unroll
@@ -357,7 +351,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         }
         double ga = adj_as + pA_s * aA;
         if constexpr (REFINE) ga = ga + (rho * (alphas[k] - ta[k]) - lam[3 + k]);
-        grad[k] = (REFINE ? grad[k] : 0.01) + ga;
+        grad[k] = (REFINE ? dist[k] : 0.01) + ga;
         aA = adj_ad + pA_d * aA;
         aB[0] = nB[0];
         aB[1] = nB[1];
@@ -365,15 +359,26 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     }
     double ga0 = aA;
     if constexpr (REFINE) ga0 = ga0 + (rho * (alphas[0] - ta[0]) - lam[3]);
-    grad[0] = (REFINE ? grad[0] : 0.01) + ga0;
     {
         const DevLayer& l0 = layer_after(lay, 0, dep);
+        dist[0] = distance(colors[0], colors[1], colors[2], l0);
+        grad[0] = (REFINE ? dist[0] : 0.01) + ga0;
         double gE[3];
         distance_gradient(colors[0], colors[1], colors[2], l0, gE);
 #pragma unroll
         for (int c = 0; c < 3; ++c) grad[L + c] = alphas[0] * gE[c] + aB[c];
     }
-    return f;
+    // the energy, its terms summed bottom to top as the reference does
+    double energy = 0.0;
+#pragma unroll
+    for (int k = 0; k < L; ++k) energy += alphas[k] * dist[k];
+    if constexpr (!REFINE) {
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < L; ++i) sum += alphas[i];
+        energy += 0.01 * sum;
+    }
+    return energy + (-lagrange) + 0.5 * rho * squared;
 }
 
 }  // namespace ubhip
