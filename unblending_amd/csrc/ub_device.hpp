@@ -315,12 +315,16 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
 
     double aB[3] = {w[0], w[1], w[2]};
     double aA = REFINE ? 0.0 : w[3 < NW ? 3 : 0];
-    double dep = w[0];
+    double dep = w[0], aprev = w[0];
     double dist[L];  // the layers' colour-model distances (the energy's terms)
 #pragma unroll
     for (int k = L - 1; k >= 1; --k) {
         const DevLayer& l = layer_after(lay, k, dep);
-        const int mode = uniform_mode(l.mode);
+        // refinement: the blend mode is read once the reverse chain has left
+        // the step two layers up, so the blend (a branch on the mode) is not
+        // hoisted, with its operands, ahead of earlier layers' steps (the
+        // first-pass kernel is faster without this; measured)
+        const int mode = uniform_mode(REFINE ? layer_after(lay, k, aprev).mode : l.mode);
         const double* cs = colors + 3 * k;
         const double* cd = pre[k - 1];
         const double as = alphas[k], ad = pre[k - 1][3];
@@ -352,6 +356,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
         double ga = adj_as + pA_s * aA;
         if constexpr (REFINE) ga = ga + (rho * (alphas[k] - ta[k]) - lam[3 + k]);
         grad[k] = (REFINE ? dist[k] : 0.01) + ga;
+        aprev = aA;
         aA = adj_ad + pA_d * aA;
         aB[0] = nB[0];
         aB[1] = nB[1];
