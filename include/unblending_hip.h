@@ -51,10 +51,15 @@ typedef struct unblending_hip_params {
                                  0: chosen by radius */
     int32_t filter_rows;      /* guided filter rows per strip; 0: chosen by image size */
     int32_t warm_outer_cap;   /* see warm_start; 0: 8 */
+    int32_t lbfgs_memory;     /* (s, y) pairs the inner minimiser keeps.  0: NLopt's default vector
+                                 storage for the reference's LD_LBFGS (unblending.cpp:249), which at
+                                 maxeval 1000 keeps every pair of an inner solve; the history lives in
+                                 the workspace.  1: one pair in registers, the fast mode (its results
+                                 differ from the default's).  n > 1: n pairs. */
 } unblending_hip_params;
 
 #define UNBLENDING_HIP_PARAMS_INIT \
-    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL, 0, 0, 0, 0 }
+    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL, 0, 0, 0, 0, 0 }
 
 typedef struct unblending_hip_stats {
     uint64_t evals;        /* objective+gradient evaluations, summed over pixels */
@@ -73,6 +78,7 @@ typedef struct unblending_hip_stats {
     double predict_ms;        /* warm start: device time of the model's prediction (timing = 1) */
     uint64_t lane_slots;      /* persistent solve: lane evaluations, useful or idle (evals / lane_slots
                                  is the share of lanes doing useful work) */
+    uint64_t history_pairs;   /* (s, y) pairs the search directions' two-loop recursions went through */
 } unblending_hip_stats;
 
 UNBLENDING_API int32_t unblending_hip_unmix_ex(
@@ -86,11 +92,19 @@ UNBLENDING_API int32_t unblending_hip_refine_ex(
     const unblending_hip_params* params, double* output, size_t output_capacity,
     unblending_hip_stats* stats);
 
-/* Bytes of device workspace the device-resident calls need (warm_start: a
- * first pass started from the model's prediction). */
+/* Bytes of device workspace the device-resident calls need with default
+ * params (warm_start: kept for callers; the prediction needs none).  The
+ * default inner minimiser keeps its (s, y) history in the workspace: up to
+ * 1024 x 64 lanes x 1000 pairs x (8 L + 2) doubles, fewer blocks for images
+ * under 65536 pixels. */
 UNBLENDING_API size_t unblending_hip_workspace_size(int32_t width, int32_t height,
                                                    size_t layer_count, int32_t refine,
                                                    int32_t smooth_background, int32_t warm_start);
+/* The same for the given params (NULL: defaults); 0 for invalid arguments. */
+UNBLENDING_API size_t unblending_hip_workspace_size_ex(int32_t width, int32_t height,
+                                                      size_t layer_count, int32_t refine,
+                                                      int32_t smooth_background,
+                                                      const unblending_hip_params* params);
 
 UNBLENDING_API int32_t unblending_hip_unmix_device(
     const double* rgb, int32_t width, int32_t height, const unblending_layer* layers,
@@ -173,6 +187,9 @@ typedef struct unblending_hip_image {
 
 UNBLENDING_API size_t unblending_hip_batch_workspace_size(const unblending_hip_image* images, size_t count,
                                                          int32_t refine, int32_t smooth_background);
+UNBLENDING_API size_t unblending_hip_batch_workspace_size_ex(const unblending_hip_image* images, size_t count,
+                                                            int32_t refine, int32_t smooth_background,
+                                                            const unblending_hip_params* params);
 UNBLENDING_API int32_t unblending_hip_unmix_batch_device(
     const unblending_hip_image* images, size_t count, const unblending_options* options,
     const unblending_hip_params* params, void* workspace, size_t workspace_size, void* stream,

@@ -51,7 +51,6 @@ extern "C" {
 #define ORC_MAX_LAYERS 16
 #define ORC_MAX_VARS (4 * ORC_MAX_LAYERS)
 #define ORC_MAX_CONSTRAINTS (3 + ORC_MAX_LAYERS)
-#define ORC_MAX_MEMORY 16
 
 /* Same field meaning as unblending_layer (unblending/include/unblending.h:67-72). */
 typedef struct orc_layer {
@@ -64,7 +63,8 @@ typedef struct orc_layer {
 /* Solver settings.  Defaults (orc_default_settings) are the reference's
  * constants, unblending.cpp:220-249, plus the inner minimiser's own. */
 typedef struct orc_settings {
-    int32_t memory;        /* L-BFGS pairs kept */
+    int32_t memory;        /* L-BFGS pairs kept; <= 0: NLopt's default vector
+                              storage (orc_lbfgs_memory), the default */
     int32_t max_evals;     /* per inner solve (reference: 1000) */
     int32_t max_count;     /* outer loop stops when count > max_count (reference: 20) */
     int32_t max_ls;        /* line-search trials per iteration */
@@ -85,9 +85,25 @@ typedef struct orc_counters {
     int64_t iterations;   /* search directions computed (two-loop recursions) */
     int64_t outer;        /* outer AL iterations (inner solves) */
     int64_t pixels;
+    int64_t max_pairs;    /* most (s, y) pairs one inner solve held */
+    int64_t pairs_used;   /* (s, y) pairs the two-loop recursions went through */
 } orc_counters;
 
 void orc_default_settings(orc_settings* s);
+/* Pairs the inner minimiser keeps for n variables: memory when > 0, else
+ * NLopt's default for LD_LBFGS, max(1310720 / n, 10) lowered to max_evals
+ * (1000 at the reference's maxeval: the whole history of an inner solve). */
+int orc_lbfgs_memory(int n, int max_evals, int memory);
+/* The inner minimiser's dot product for its two-loop recursion: partial sums
+ * p_j of components [w j, w j + w) (w = 4 for n <= 32, else ceil(n / 8)),
+ * each summed in index order from 0.0, then
+ * ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)). */
+double orc_tree_dot(const double* a, const double* b, int n);
+/* The two-loop recursion alone: d = H q for count (s, y) pairs given oldest
+ * first (S, Y: count x n, row-major), rho_k = 1 / (s_k . y_k) in index order,
+ * H0 = gamma I.  Exposed to pin the recursion. */
+void orc_lbfgs_direction(const double* S, const double* Y, int count, double gamma, const double* q, int n,
+                         double* d);
 
 /* ---- pixel math ---- */
 double orc_blend(double s, double d, int mode);
@@ -134,11 +150,13 @@ int orc_solve_pixel(const double target[3], int L, const orc_layer* layers, int 
 int orc_solve_pixel_warm(const double target[3], int L, const orc_layer* layers, int opaque,
                          const double* x_init, const double* lambda_init, const orc_settings* settings,
                          double* x_out, orc_counters* counters);
-/* The inner minimiser alone (exposed for its invariant tests). */
+/* The inner minimiser alone (exposed for its invariant tests).  Returns the
+ * most (s, y) pairs the history held; *pairs_used_out (may be NULL) receives
+ * the pairs its two-loop recursions went through. */
 typedef double (*orc_fn)(const double* x, double* grad, void* data);
 int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn, void* data,
                  const orc_settings* settings, double* f_out, int32_t* evals_out,
-                 int32_t* iterations_out);
+                 int32_t* iterations_out, int64_t* pairs_used_out);
 
 /* ---- filter ---- */
 /* Box filter, clamp-edge, radius r.  brute = 1 uses the reference's

@@ -52,6 +52,8 @@ class OrcCounters(ctypes.Structure):
         ("iterations", ctypes.c_int64),
         ("outer", ctypes.c_int64),
         ("pixels", ctypes.c_int64),
+        ("max_pairs", ctypes.c_int64),
+        ("pairs_used", ctypes.c_int64),
     ]
 
 
@@ -94,6 +96,11 @@ def lib():
         L.orc_objective.restype = ctypes.c_double
         L.orc_objective.argtypes = [dp, ctypes.c_int, lp, dp, dp, dp, ctypes.c_double, dp, dp]
         L.orc_default_settings.argtypes = [ctypes.POINTER(OrcSettings)]
+        L.orc_tree_dot.restype = ctypes.c_double
+        L.orc_tree_dot.argtypes = [dp, dp, ctypes.c_int]
+        L.orc_lbfgs_direction.argtypes = [dp, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, dp]
+        L.orc_lbfgs_memory.restype = ctypes.c_int
+        L.orc_lbfgs_memory.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_solve_pixel.restype = ctypes.c_int
         L.orc_solve_pixel.argtypes = [dp, ctypes.c_int, lp, ctypes.c_int, dp, dp, dp,
                                       ctypes.POINTER(OrcSettings), dp, ctypes.POINTER(OrcCounters)]
@@ -103,7 +110,7 @@ def lib():
         L.orc_minimize.restype = ctypes.c_int
         L.orc_minimize.argtypes = [dp, dp, dp, ctypes.c_int, ORC_FN, ctypes.c_void_p,
                                    ctypes.POINTER(OrcSettings), dp, ctypes.POINTER(ctypes.c_int32),
-                                   ctypes.POINTER(ctypes.c_int32)]
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
         L.orc_box.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_guided_filter.argtypes = [dp, dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
@@ -266,6 +273,26 @@ def solve_pixel_warm(target, layers, x_init, lambda_init=None, opaque=True, sett
     return out, cnt, bool(capped)
 
 
+def tree_dot(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return lib().orc_tree_dot(_dp(a), _dp(b), a.size)
+
+
+def lbfgs_direction(S, Y, gamma, q):
+    """H q from the pairs (rows of S, Y, oldest first) by the minimiser's two-loop recursion."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    d = np.zeros_like(q)
+    lib().orc_lbfgs_direction(_dp(S), _dp(Y), S.shape[0], float(gamma), _dp(q), q.size, _dp(d))
+    return d
+
+
+def lbfgs_memory(n, max_evals=1000, memory=0):
+    return lib().orc_lbfgs_memory(int(n), int(max_evals), int(memory))
+
+
 def minimize(fn, x0, lb, ub, settings=None):
     """Run the inner minimiser on a Python objective fn(x) -> (f, grad)."""
     x = np.ascontiguousarray(x0, dtype=np.float64).copy()
@@ -288,7 +315,7 @@ def minimize(fn, x0, lb, ub, settings=None):
     it = ctypes.c_int32()
     st = settings or default_settings()
     lib().orc_minimize(_dp(x), _dp(lb), _dp(ub), n, cfn, None, ctypes.byref(st), ctypes.byref(f),
-                       ctypes.byref(ev), ctypes.byref(it))
+                       ctypes.byref(ev), ctypes.byref(it), None)
     return x, f.value, ev.value, it.value, trace
 
 

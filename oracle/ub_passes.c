@@ -39,7 +39,7 @@ static void* pass_worker(void* arg)
     pass_job* job = (pass_job*)arg;
     const size_t n = (size_t)job->w * job->h;
     const int L = job->L;
-    orc_counters local = {0, 0, 0, 0};
+    orc_counters local = {0, 0, 0, 0, 0, 0};
     for (;;) {
         pthread_mutex_lock(&job->lock);
         const int y = job->next_row++;
@@ -79,6 +79,8 @@ static void* pass_worker(void* arg)
     job->totals.iterations += local.iterations;
     job->totals.outer += local.outer;
     job->totals.pixels += local.pixels;
+    job->totals.pairs_used += local.pairs_used;
+    if (local.max_pairs > job->totals.max_pairs) job->totals.max_pairs = local.max_pairs;
     pthread_mutex_unlock(&job->lock);
     return NULL;
 }
@@ -101,6 +103,8 @@ static void run_pass(pass_job* job, int threads, orc_counters* counters)
         counters->iterations += job->totals.iterations;
         counters->outer += job->totals.outer;
         counters->pixels += job->totals.pixels;
+        counters->pairs_used += job->totals.pairs_used;
+        if (job->totals.max_pairs > counters->max_pairs) counters->max_pairs = job->totals.max_pairs;
     }
 }
 

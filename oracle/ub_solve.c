@@ -15,7 +15,10 @@
  *   2. direction: the two-loop recursion over the stored (s, y) pairs
  *      (newest to oldest, then back), initial scaling s.y / y.y of the newest
  *      pair; d = -H q on the free set, 0 elsewhere.  If g.d is not negative the
- *      history is dropped and d = -q.
+ *      history is dropped and d = -q.  The recursion's dot products are summed
+ *      in a fixed tree order (orc_tree_dot): eight partial sums of four
+ *      consecutive components each, then pairwise -- the order in which the
+ *      device's 8-lane groups form them.
  *   3. step: t = 1, or min(1, 1/|d|) with an empty history.  Trial points
  *      x(t) = clamp(x + t d, lb, ub); accept when f(x(t)) <= f + 1e-4 g.(x(t) - x).
  *      Otherwise t <- the minimiser of the quadratic through f, g.(x(t)-x) and
@@ -24,6 +27,12 @@
  *      accepted point (the monotone minimiser's best point).
  *   4. update: s = x+ - x, y = g+ - g; the pair is kept when s.y > 2^-52 y.y
  *      (memory pairs, oldest dropped), otherwise the whole history is dropped.
+ *      memory <= 0 selects NLopt's default vector storage for LD_LBFGS:
+ *      M = max(MEMAVAIL / n, 10) with MEMAVAIL = 1310720 doubles, lowered to
+ *      maxeval when maxeval <= M (NLopt src/algs/luksan/plis.c, luksan.h; the
+ *      reference never calls set_vector_storage).  At n <= 64 and maxeval =
+ *      1000 that is M = 1000: no pair of an inner solve is ever dropped,
+ *      since every iteration costs at least one evaluation.
  *   5. stop (NLopt's relative tests, nlopt util/stop.c semantics):
  *      |f+ - f| < ftol_rel (|f+| + |f|)/2 or f+ == f; or for every i
  *      |x+_i - x_i| < xtol_rel (|x+_i| + |x_i|)/2 or x+_i == x_i; or the
@@ -34,11 +43,12 @@
 #include "ub_oracle.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 void orc_default_settings(orc_settings* s)
 {
-    s->memory = 1;
+    s->memory = 0;         /* NLopt's default vector storage (orc_lbfgs_memory) */
     s->max_evals = 1000;   /* unblending.cpp:249 */
     s->max_count = 20;     /* unblending.cpp:246 */
     s->max_ls = 20;
@@ -69,16 +79,119 @@ static int relstop(double vold, double vnew, double reltol)
     return fabs(vnew - vold) < reltol * (fabs(vnew) + fabs(vold)) * 0.5 || vnew == vold;
 }
 
-int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn, void* data,
-                 const orc_settings* st, double* f_out, int32_t* evals_out, int32_t* iterations_out)
+/* components per partial sum of orc_tree_dot */
+static int tree_width(int n) { return n <= 32 ? 4 : (n + 7) / 8; }
+
+double orc_tree_dot(const double* a, const double* b, int n)
 {
-    const int M = st->memory < 1 ? 1 : (st->memory > ORC_MAX_MEMORY ? ORC_MAX_MEMORY : st->memory);
+    const int w = tree_width(n);
+    double p[8];
+    for (int j = 0; j < 8; ++j) {
+        double s = 0.0;
+        for (int c = 0; c < w; ++c) {
+            const int i = j * w + c;
+            if (i < n) s = s + a[i] * b[i];
+        }
+        p[j] = s;
+    }
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+}
+
+int orc_lbfgs_memory(int n, int max_evals, int memory)
+{
+    if (memory > 0) return memory;
+    int m = 1310720 / (n > 0 ? n : 1);
+    if (m < 10) m = 10;
+    if (max_evals > 0 && max_evals <= m) m = max_evals;
+    return m;
+}
+
+/* (s, y) history of one inner solve: a ring of M pairs, held on the stack
+ * while it is short and moved to the heap when it outgrows that. */
+#define ORC_STACK_PAIRS 32
+typedef struct {
+    double* S;  /* pair k of the ring: S + k * n */
+    double* Y;
+    double* rh;
+    double* a;
+    int cap, n;
+    void* heap;
+} lbfgs_history;
+
+static int history_grow(lbfgs_history* h, int head, int count, int want)
+{
+    const size_t pairs = (size_t)want;
+    double* mem = (double*)malloc(sizeof(double) * pairs * (2 * (size_t)h->n + 2));
+    if (!mem) return 0;
+    double* S = mem;
+    double* Y = S + pairs * h->n;
+    double* rh = Y + pairs * h->n;
+    double* a = rh + pairs;
+    for (int k = 0; k < count; ++k) { /* oldest first */
+        const int j = (head + k) % h->cap;
+        memcpy(S + (size_t)k * h->n, h->S + (size_t)j * h->n, sizeof(double) * h->n);
+        memcpy(Y + (size_t)k * h->n, h->Y + (size_t)j * h->n, sizeof(double) * h->n);
+        rh[k] = h->rh[j];
+    }
+    free(h->heap);
+    h->heap = mem;
+    h->S = S;
+    h->Y = Y;
+    h->rh = rh;
+    h->a = a;
+    h->cap = want;
+    return 1;
+}
+
+/* d = H q for the history's count pairs (the k-th oldest in slot
+ * (head + k) % cap), H the L-BFGS inverse-Hessian approximation from
+ * H0 = gamma I; q is overwritten. */
+static void two_loop(lbfgs_history* hs, int head, int count, double gamma, double* q, double* d, int n)
+{
+    for (int k = count - 1; k >= 0; --k) {
+        const int j = (head + k) % hs->cap;
+        const double* Sj = hs->S + (size_t)j * n;
+        const double* Yj = hs->Y + (size_t)j * n;
+        hs->a[k] = hs->rh[j] * orc_tree_dot(Sj, q, n);
+        for (int i = 0; i < n; ++i) q[i] = q[i] - hs->a[k] * Yj[i];
+    }
+    for (int i = 0; i < n; ++i) d[i] = gamma * q[i];
+    for (int k = 0; k < count; ++k) {
+        const int j = (head + k) % hs->cap;
+        const double* Sj = hs->S + (size_t)j * n;
+        const double* Yj = hs->Y + (size_t)j * n;
+        const double b = hs->rh[j] * orc_tree_dot(Yj, d, n);
+        for (int i = 0; i < n; ++i) d[i] = d[i] + (hs->a[k] - b) * Sj[i];
+    }
+}
+
+void orc_lbfgs_direction(const double* S, const double* Y, int count, double gamma, const double* q, int n,
+                         double* d)
+{
+    double* rh = (double*)malloc(sizeof(double) * (size_t)(2 * (count > 0 ? count : 1)));
+    double qq[ORC_MAX_VARS];
+    if (!rh) abort();
+    for (int k = 0; k < count; ++k) rh[k] = 1.0 / dot(S + (size_t)k * n, Y + (size_t)k * n, n);
+    for (int i = 0; i < n; ++i) qq[i] = q[i];
+    lbfgs_history hs = {(double*)S, (double*)Y, rh, rh + count, count > 0 ? count : 1, n, NULL};
+    two_loop(&hs, 0, count, gamma, qq, d, n);
+    free(rh);
+}
+
+int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn, void* data,
+                 const orc_settings* st, double* f_out, int32_t* evals_out, int32_t* iterations_out,
+                 int64_t* pairs_used_out)
+{
+    const int M = orc_lbfgs_memory(n, st->max_evals, st->memory);
     double g[ORC_MAX_VARS], d[ORC_MAX_VARS], q[ORC_MAX_VARS], xt[ORC_MAX_VARS], gt[ORC_MAX_VARS];
-    double S[ORC_MAX_MEMORY][ORC_MAX_VARS], Yh[ORC_MAX_MEMORY][ORC_MAX_VARS];
-    double rh[ORC_MAX_MEMORY], a[ORC_MAX_MEMORY];
+    double S0[ORC_STACK_PAIRS * ORC_MAX_VARS], Y0[ORC_STACK_PAIRS * ORC_MAX_VARS];
+    double rh0[ORC_STACK_PAIRS], a0[ORC_STACK_PAIRS];
+    lbfgs_history hs = {S0, Y0, rh0, a0, M < ORC_STACK_PAIRS ? M : ORC_STACK_PAIRS, n, NULL};
     int freev[ORC_MAX_VARS];
-    int count = 0, head = 0; /* ring: slot (head + k) % M holds the k-th oldest pair */
+    int count = 0, head = 0; /* ring: slot (head + k) % cap holds the k-th oldest pair */
     double gamma = 1.0;
+    int32_t max_run = 0;
+    int64_t used = 0;
 
     for (int i = 0; i < n; ++i) x[i] = clampd(x[i], lb[i], ub[i]);
     double f = fn(x, g, data);
@@ -98,19 +211,9 @@ int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn
         }
         if (pgmax == 0.0) break;
         ++iters;
+        used += count;
 
-        /* two-loop recursion */
-        for (int k = count - 1; k >= 0; --k) {
-            const int j = (head + k) % M;
-            a[k] = rh[j] * dot(S[j], q, n);
-            for (int i = 0; i < n; ++i) q[i] = q[i] - a[k] * Yh[j][i];
-        }
-        for (int i = 0; i < n; ++i) d[i] = gamma * q[i];
-        for (int k = 0; k < count; ++k) {
-            const int j = (head + k) % M;
-            const double b = rh[j] * dot(Yh[j], d, n);
-            for (int i = 0; i < n; ++i) d[i] = d[i] + (a[k] - b) * S[j][i];
-        }
+        two_loop(&hs, head, count, gamma, q, d, n);
         for (int i = 0; i < n; ++i) d[i] = freev[i] ? -d[i] : 0.0;
         double gd = dot(g, d, n);
         if (!(gd < 0.0)) {
@@ -153,28 +256,37 @@ int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn
         }
         if (!accepted) break;
 
-        /* update */
+        /* update: a full ring below M pairs grows (heap) instead of wrapping */
+        if (count == hs.cap && hs.cap < M) {
+            int want = 4 * hs.cap;
+            if (want > M) want = M;
+            if (!history_grow(&hs, head, count, want)) abort();
+            head = 0;
+        }
         double sy = 0.0, yy = 0.0;
         int xstop = 1;
-        const int slot = (count < M) ? (head + count) % M : head;
+        const int slot = (count < hs.cap) ? (head + count) % hs.cap : head;
+        double* Ss = hs.S + (size_t)slot * n;
+        double* Ys = hs.Y + (size_t)slot * n;
         for (int i = 0; i < n; ++i) {
             const double s = xt[i] - x[i];
             const double y = gt[i] - g[i];
             sy += s * y;
             yy += y * y;
             xstop = xstop && (fabs(s) < st->xtol_rel * (fabs(xt[i]) + fabs(x[i])) * 0.5 || xt[i] == x[i]);
-            S[slot][i] = s;
-            Yh[slot][i] = y;
+            Ss[i] = s;
+            Ys[i] = y;
             x[i] = xt[i];
             g[i] = gt[i];
         }
         const double fold = f;
         f = ft;
         if (sy > 2.220446049250313e-16 * yy) {
-            rh[slot] = 1.0 / sy;
+            hs.rh[slot] = 1.0 / sy;
             gamma = sy / yy;
-            if (count < M) ++count;
-            else head = (head + 1) % M;
+            if (count < hs.cap) ++count;
+            else head = (head + 1) % hs.cap;
+            if (count > max_run) max_run = count;
         } else {
             count = 0;
             head = 0;
@@ -184,10 +296,12 @@ int orc_minimize(double* x, const double* lb, const double* ub, int n, orc_fn fn
         if (relstop(fold, f, st->ftol_rel)) break;
         if (xstop) break;
     }
+    free(hs.heap);
     if (f_out) *f_out = f;
     if (evals_out) *evals_out = nev;
     if (iterations_out) *iterations_out = iters;
-    return 0;
+    if (pairs_used_out) *pairs_used_out = used;
+    return max_run;
 }
 
 typedef struct {
@@ -286,11 +400,14 @@ static int solve_pixel_from(const double target[3], int L, const orc_layer* laye
     for (;;) {
         memcpy(xn, x, sizeof(double) * (size_t)N);
         int32_t ev = 0, it = 0;
-        orc_minimize(xn, lb, ub, N, pixel_fn, &p, st, NULL, &ev, &it);
+        int64_t used = 0;
+        const int pairs = orc_minimize(xn, lb, ub, N, pixel_fn, &p, st, NULL, &ev, &it, &used);
         if (counters) {
+            counters->pairs_used += used;
             counters->evals += ev;
             counters->iterations += it;
             counters->outer += 1;
+            if (pairs > counters->max_pairs) counters->max_pairs = pairs;
         }
         double g[ORC_MAX_CONSTRAINTS], gn[ORC_MAX_CONSTRAINTS], dx[ORC_MAX_VARS];
         orc_constraints(x, L, layers, target, target_alphas, g);

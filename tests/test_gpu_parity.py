@@ -31,17 +31,23 @@ def _workload(k, size):
     return synth.config(k, size)
 
 
-SOLVERS = [0, 1]  # persistent kernel, one-pixel-per-lane kernel
+# (solver kernel, L-BFGS memory): the persistent kernel with the default
+# history (NLopt's default vector storage: every pair of an inner solve), with
+# a 3-pair ring that wraps, and with the one-pair register minimiser (the fast
+# mode); the one-pixel-per-lane kernel (one pair)
+MODES = [(0, 0), (0, 3), (0, 1), (1, 1)]
 
 
-@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("solver,memory", MODES)
 @pytest.mark.parametrize("k,size", [(1, (64, 48)), (2, (40, 40)), (3, (32, 36))])
-def test_unmix_matches_oracle(oracle, k, size, solver):
+def test_unmix_matches_oracle(oracle, k, size, solver, memory):
     import unblending_amd as U
     wl = _workload(k, size)
-    print("case unmix c%d %s solver %d" % (k, size, solver), flush=True)
-    gpu = U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True, params=U.hip_params(solver=solver))
-    ref, _ = oracle.unmix(wl.rgb, oracle_layers(wl.layers), opaque=True)
+    print("case unmix c%d %s solver %d memory %d" % (k, size, solver, memory), flush=True)
+    gpu = U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True,
+                                   params=U.hip_params(solver=solver, lbfgs_memory=memory))
+    ref, _ = oracle.unmix(wl.rgb, oracle_layers(wl.layers), opaque=True,
+                          settings=oracle.default_settings(memory=memory))
     d = _report("unmix c%d" % k, gpu, ref)
     assert d.max() <= ALPHA_COLOR_TOL
     assert d.max() == 0.0
@@ -50,17 +56,18 @@ def test_unmix_matches_oracle(oracle, k, size, solver):
     assert np.sqrt(np.mean((rg - rr) ** 2)) <= RECOMPOSITE_RMS_TOL
 
 
-@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("solver,memory", MODES)
 @pytest.mark.parametrize("k,size", [(1, (64, 48)), (2, (40, 40)), (3, (32, 36))])
-def test_refine_matches_oracle(oracle, k, size, solver):
+def test_refine_matches_oracle(oracle, k, size, solver, memory):
     import unblending_amd as U
     wl = _workload(k, size)
-    print("case refine c%d %s solver %d" % (k, size, solver), flush=True)
+    print("case refine c%d %s solver %d memory %d" % (k, size, solver, memory), flush=True)
     ol = oracle_layers(wl.layers)
-    first, _ = oracle.unmix(wl.rgb, ol, opaque=True)
-    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=solver)
+    st = oracle.default_settings(memory=memory)
+    first, _ = oracle.unmix(wl.rgb, ol, opaque=True, settings=st)
+    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=solver, lbfgs_memory=memory)
     gpu = U.perform_matte_refinement(wl.rgb, first, layer_infos(wl.layers), True, False, params=params)
-    ref, _ = oracle.refine(wl.rgb, ol, first, opaque=True, radius=wl.radius, eps=wl.eps)
+    ref, _ = oracle.refine(wl.rgb, ol, first, opaque=True, radius=wl.radius, eps=wl.eps, settings=st)
     d = _report("refine c%d" % k, gpu, ref)
     assert d.max() <= ALPHA_COLOR_TOL
     assert d.max() == 0.0

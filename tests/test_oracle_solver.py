@@ -6,7 +6,15 @@ NLopt iterates are not reproducible here, so it is pinned by invariants):
     separable bounded quadratic,
   - every evaluated point lies inside the bounds,
   - the returned value never exceeds the starting value,
-  - the evaluation cap is honoured.
+  - the evaluation cap is honoured,
+  - its two-loop recursion equals the dense BFGS inverse-Hessian recursion
+    H <- V^T H V + rho s s^T from H0 = gamma I,
+  - an independent Python transcription of the documented method (free set,
+    history ring, line search, stopping tests) reproduces it bit for bit at
+    several memory sizes, including a ring that wraps and one that outgrows
+    its first storage,
+  - the default memory is NLopt's default vector storage for LD_LBFGS
+    (1000 pairs at maxeval 1000): it keeps every pair of an inner solve.
 Outer loop (unblending.cpp:155-309): an independent Python transcription of
 the reference loop (init, bounds, lambda/rho updates, stopping rule) built on
 the same minimiser and objective must reproduce orc_solve_pixel bit for bit.
@@ -51,6 +59,159 @@ def test_minimizer_rosenbrock_monotone_and_capped(oracle):
     st = oracle.default_settings(max_evals=7)
     x, f, ev, it, trace = oracle.minimize(fn, np.array([-1.2, 1.0]), lb, ub, st)
     assert ev <= 7 and f <= trace[0][1]
+
+
+def dense_bfgs(S, Y, gamma, q):
+    """H q with H built by the textbook BFGS update of the inverse Hessian."""
+    n = q.size
+    H = gamma * np.eye(n)
+    for s, y in zip(S, Y):
+        rho = 1.0 / np.dot(s, y)
+        V = np.eye(n) - rho * np.outer(y, s)
+        H = V.T @ H @ V + rho * np.outer(s, s)
+    return H @ q
+
+
+def test_two_loop_equals_dense_bfgs(oracle):
+    rng = np.random.default_rng(3)
+    for n, count in [(12, 1), (20, 5), (28, 17), (28, 40)]:
+        A = rng.normal(size=(n, n))
+        A = A @ A.T + n * np.eye(n)          # SPD: every pair has s.y > 0
+        S = rng.normal(size=(count, n))
+        Y = S @ A
+        q = rng.normal(size=n)
+        gamma = float(np.dot(S[-1], Y[-1]) / np.dot(Y[-1], Y[-1]))
+        d = oracle.lbfgs_direction(S, Y, gamma, q)
+        want = dense_bfgs(S, Y, gamma, q)
+        assert np.abs(d - want).max() <= 1e-10 * np.abs(want).max()
+        if count > 1:
+            # a plausible slip -- pairs taken newest first -- fails the check
+            wrong = dense_bfgs(S[::-1], Y[::-1], gamma, q)
+            assert np.abs(wrong - want).max() > 1e-6 * np.abs(want).max()
+
+
+def test_tree_dot(oracle):
+    rng = np.random.default_rng(4)
+    for n in (1, 5, 12, 20, 28, 32, 40, 64):
+        a = rng.integers(-1000, 1000, n).astype(np.float64)
+        b = rng.integers(-1000, 1000, n).astype(np.float64)
+        assert oracle.tree_dot(a, b) == float(np.dot(a, b))  # exact sums: any order agrees
+        a = rng.normal(size=n)
+        assert abs(oracle.tree_dot(a, a) - np.dot(a, a)) <= 1e-13 * np.dot(a, a)
+
+
+def py_minimize(oracle, fn, x0, lb, ub, memory, max_evals=1000, max_ls=20, ftol=5e-3, xtol=5e-3):
+    """The documented inner minimiser (ub_solve.c header, steps 1-5), retold."""
+    n = x0.size
+    M = oracle.lbfgs_memory(n, max_evals, memory)
+
+    def seq_dot(a, b):
+        s = 0.0
+        for i in range(n):
+            s = s + float(a[i]) * float(b[i])
+        return s
+
+    x = np.clip(x0, lb, ub)
+    f, g = fn(x)
+    nev = 1
+    hist = []          # oldest first, at most M pairs
+    gamma = 1.0
+    while True:
+        if not (np.isfinite(f) and np.all(np.isfinite(g))):
+            break
+        free = (lb < ub) & ~((x <= lb) & (g > 0)) & ~((x >= ub) & (g < 0))
+        q = np.where(free, g, 0.0)
+        if np.abs(q).max() == 0.0:
+            break
+        alphas = []
+        for s_, y_, r_ in reversed(hist):
+            a = r_ * oracle.tree_dot(s_, q)
+            alphas.append(a)
+            q = q - a * y_
+        d = gamma * q
+        for (s_, y_, r_), a in zip(hist, reversed(alphas)):
+            b = r_ * oracle.tree_dot(y_, d)
+            d = d + (a - b) * s_
+        d = np.where(free, -d, 0.0)
+        if not (seq_dot(g, d) < 0.0):
+            hist, gamma = [], 1.0
+            d = np.where(free, -g, 0.0)
+        t = 1.0
+        if not hist:
+            t = min(1.0, 1.0 / np.sqrt(seq_dot(d, d)))
+        accepted = False
+        ls = 0
+        while True:
+            xt = np.clip(x + t * d, lb, ub)
+            ft, gt = fn(xt)
+            nev += 1
+            dphi = seq_dot(g, xt - x)
+            if np.isfinite(ft) and ft <= f + 1e-4 * dphi:
+                accepted = True
+                break
+            if nev >= max_evals or ls + 1 >= max_ls:
+                break
+            tn = (-dphi * t) / (2.0 * (ft - f - dphi)) if dphi < 0.0 else 0.5 * t
+            tn = tn if tn >= 0.1 * t else 0.1 * t
+            t = min(tn, 0.5 * t)
+            ls += 1
+        if not accepted:
+            break
+        s_, y_ = xt - x, gt - g
+        sy, yy = seq_dot(s_, y_), seq_dot(y_, y_)
+        xstop = bool(np.all((np.abs(s_) < xtol * (np.abs(xt) + np.abs(x)) * 0.5) | (xt == x)))
+        fold = f
+        x, g, f = xt, gt, ft
+        if sy > 2.220446049250313e-16 * yy:
+            hist.append((s_, y_, 1.0 / sy))
+            if len(hist) > M:
+                hist.pop(0)
+            gamma = sy / yy
+        else:
+            hist, gamma = [], 1.0
+        if nev >= max_evals:
+            break
+        if abs(f - fold) < ftol * (abs(f) + abs(fold)) * 0.5 or f == fold:
+            break
+        if xstop:
+            break
+    return x, f, nev
+
+
+@pytest.mark.parametrize("memory", [1, 3, 40, 0])
+def test_minimizer_matches_transcription(oracle, memory):
+    rng = np.random.default_rng(6)
+    n = 28
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A = (Q * np.logspace(-3, 2, n)) @ Q.T   # ill-conditioned: long inner runs
+    c = rng.uniform(0.1, 0.9, n)
+
+    def fn(x):   # a bounded quadratic with a mild quartic
+        r = x - c
+        return 0.5 * r @ A @ r + 0.25 * np.sum(r ** 4), A @ r + r ** 3
+
+    lb, ub = np.zeros(n), np.ones(n)
+    lb[3] = ub[3] = 0.25
+    st = oracle.default_settings(memory=memory, ftol_rel=1e-15, xtol_rel=1e-15, max_evals=400)
+    x, f, ev, it, _ = oracle.minimize(fn, np.full(n, 0.5), lb, ub, st)
+    xw, fw, evw = py_minimize(oracle, fn, np.full(n, 0.5), lb, ub, memory, max_evals=400, ftol=1e-15, xtol=1e-15)
+    assert ev == evw and f == fw and np.array_equal(x, xw)
+    # the runs are long enough to wrap the small rings (3, 40 pairs) and to
+    # outgrow the minimiser's first 32-pair storage (default memory)
+    assert it > 100
+
+
+def test_default_memory_keeps_every_pair(oracle):
+    from unblending_amd import synth
+    assert oracle.lbfgs_memory(28) == 1000 and oracle.lbfgs_memory(12) == 1000
+    wl = synth.config(3, (16, 4))
+    layers = oracle_layers(wl.layers)
+    out0, c0 = oracle.unmix(wl.rgb, layers)
+    outk, ck = oracle.unmix(wl.rgb, layers, settings=oracle.default_settings(memory=1000))
+    out1, _ = oracle.unmix(wl.rgb, layers, settings=oracle.default_settings(memory=1))
+    assert np.array_equal(out0, outk) and c0.evals == ck.evals
+    assert c0.max_pairs > 1
+    assert np.abs(out0 - out1).max() > 1e-3   # the one-pair stand-in is a different method
 
 
 def al_transcription(oracle, target, layers, opaque, init_colors=None, target_alphas=None, bg=None):

@@ -151,7 +151,8 @@ class HipParams(ctypes.Structure):
                 ("filter_epsilon", ctypes.c_double), ("alpha_fixed", ctypes.c_int32),
                 ("solver", ctypes.c_int32), ("timing", ctypes.c_int32), ("warm_start", ctypes.c_int32),
                 ("mlp", ctypes.c_void_p), ("chunk_size", ctypes.c_int32), ("filter_block", ctypes.c_int32),
-                ("filter_rows", ctypes.c_int32), ("warm_outer_cap", ctypes.c_int32)]
+                ("filter_rows", ctypes.c_int32), ("warm_outer_cap", ctypes.c_int32),
+                ("lbfgs_memory", ctypes.c_int32)]
 
 
 class HipImage(ctypes.Structure):
@@ -169,7 +170,7 @@ class HipStats(ctypes.Structure):
                 ("filter_radius", ctypes.c_int32), ("filter_channels", ctypes.c_int32),
                 ("filter_launches", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("fallback_pixels", ctypes.c_uint64), ("predict_ms", ctypes.c_double),
-                ("lane_slots", ctypes.c_uint64)]
+                ("lane_slots", ctypes.c_uint64), ("history_pairs", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -196,6 +197,9 @@ _lib.unblending_hip_refine_ex.argtypes = [ctypes.POINTER(RgbView), ctypes.POINTE
 _lib.unblending_hip_workspace_size.restype = ctypes.c_size_t
 _lib.unblending_hip_workspace_size.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_size_t,
                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+_lib.unblending_hip_workspace_size_ex.restype = ctypes.c_size_t
+_lib.unblending_hip_workspace_size_ex.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_size_t,
+                                                  ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(HipParams)]
 _lib.unblending_hip_unmix_device.restype = ctypes.c_int32
 _lib.unblending_hip_unmix_device.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CLayer),
                                              ctypes.c_size_t, ctypes.POINTER(Options),
@@ -240,6 +244,9 @@ _lib.unblending_hip_mlp_predict.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.c_i
 _lib.unblending_hip_batch_workspace_size.restype = ctypes.c_size_t
 _lib.unblending_hip_batch_workspace_size.argtypes = [ctypes.POINTER(HipImage), ctypes.c_size_t, ctypes.c_int32,
                                                      ctypes.c_int32]
+_lib.unblending_hip_batch_workspace_size_ex.restype = ctypes.c_size_t
+_lib.unblending_hip_batch_workspace_size_ex.argtypes = [ctypes.POINTER(HipImage), ctypes.c_size_t, ctypes.c_int32,
+                                                        ctypes.c_int32, ctypes.POINTER(HipParams)]
 for _f in (_lib.unblending_hip_unmix_batch_device, _lib.unblending_hip_refine_batch_device):
     _f.restype = ctypes.c_int32
     _f.argtypes = [ctypes.POINTER(HipImage), ctypes.c_size_t, ctypes.POINTER(Options), ctypes.POINTER(HipParams),
@@ -296,10 +303,12 @@ def options(backend=BACKEND_AUTO, opaque=True, smooth=False, target_concurrency=
 
 
 def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, warm_start=None, chunk_size=0,
-               filter_block=0, filter_rows=0, warm_outer_cap=0) -> HipParams:
+               filter_block=0, filter_rows=0, warm_outer_cap=0, lbfgs_memory=0) -> HipParams:
     """warm_start: a trained ``mlp.WarmStartModel`` (first pass only), or None.
     chunk_size / filter_block / filter_rows: launch shapes (0: defaults; see
-    ``unblending_amd.autotune``)."""
+    ``unblending_amd.autotune``).  lbfgs_memory: (s, y) pairs of the inner
+    minimiser; 0 = NLopt's default for LD_LBFGS (every pair of an inner
+    solve), 1 = the one-pair fast mode."""
     p = HipParams()
     p.struct_size = ctypes.sizeof(HipParams)
     p.filter_radius = int(radius)
@@ -313,6 +322,7 @@ def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, wa
     p.filter_block = int(filter_block)
     p.filter_rows = int(filter_rows)
     p.warm_outer_cap = int(warm_outer_cap)
+    p.lbfgs_memory = int(lbfgs_memory)
     return p
 
 

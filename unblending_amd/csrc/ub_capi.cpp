@@ -145,12 +145,10 @@ int32_t solve(const unblending_rgb_view* image, const unblending_layer* layers, 
             (options->smooth_background != 0 && options->smooth_background != 1) ||
             options->target_concurrency < 0)
             return UNBLENDING_INVALID_ARGUMENT;
-        unblending_hip_params p = UNBLENDING_HIP_PARAMS_INIT;
-        if (params) {
-            if (params->struct_size < sizeof(unblending_hip_params)) return UNBLENDING_INVALID_ARGUMENT;
-            p = *params;
-        }
+        unblending_hip_params p;
+        if (!read_params(params, p)) return UNBLENDING_INVALID_ARGUMENT;
 
+        if (p.lbfgs_memory < 0) return UNBLENDING_INVALID_ARGUMENT;
         const Backend backend = select_backend(options->backend);
         const LayerSet set = make_layer_set(layers, layer_count);
         const bool opaque = options->opaque_background != 0;
@@ -212,11 +210,8 @@ int32_t device_call(const double* rgb, int32_t width, int32_t height, const unbl
             (options->smooth_background != 0 && options->smooth_background != 1) ||
             options->target_concurrency < 0)
             return UNBLENDING_INVALID_ARGUMENT;
-        unblending_hip_params p = UNBLENDING_HIP_PARAMS_INIT;
-        if (params) {
-            if (params->struct_size < sizeof(unblending_hip_params)) return UNBLENDING_INVALID_ARGUMENT;
-            p = *params;
-        }
+        unblending_hip_params p;
+        if (!read_params(params, p)) return UNBLENDING_INVALID_ARGUMENT;
         const Backend backend = select_backend(options->backend);
         if (backend == Backend::cpu) return UNBLENDING_INVALID_ARGUMENT;
         if (backend == Backend::cuda) return UNBLENDING_DEVICE_ERROR;
@@ -227,8 +222,9 @@ int32_t device_call(const double* rgb, int32_t width, int32_t height, const unbl
             if (smooth && !opaque) return UNBLENDING_INVALID_ARGUMENT;
             if (!set.all_plus && !set.all_so) return UNBLENDING_INVALID_ARGUMENT;
         }
-        const size_t need = unblending_hip_workspace_size(width, height, layer_count, first != nullptr,
-                                                          smooth && first != nullptr, p.warm_start);
+        if (p.lbfgs_memory < 0) return UNBLENDING_INVALID_ARGUMENT;
+        const size_t need = workspace_bytes(size_t(width) * size_t(height), layer_count, first != nullptr,
+                                            smooth && first != nullptr, p);
         if (!workspace || workspace_size < need) return UNBLENDING_BUFFER_TOO_SMALL;
         DeviceJob job;
         job.w = width;
@@ -280,12 +276,10 @@ int32_t batch_call(const unblending_hip_image* images, size_t count, const unble
             (options->smooth_background != 0 && options->smooth_background != 1) ||
             options->target_concurrency < 0)
             return UNBLENDING_INVALID_ARGUMENT;
-        unblending_hip_params p = UNBLENDING_HIP_PARAMS_INIT;
-        if (params) {
-            if (params->struct_size < sizeof(unblending_hip_params)) return UNBLENDING_INVALID_ARGUMENT;
-            p = *params;
-        }
-        if (p.warm_start != 0 || p.solver != 0 || p.chunk_size < 0) return UNBLENDING_INVALID_ARGUMENT;
+        unblending_hip_params p;
+        if (!read_params(params, p)) return UNBLENDING_INVALID_ARGUMENT;
+        if (p.warm_start != 0 || p.solver != 0 || p.chunk_size < 0 || p.lbfgs_memory < 0)
+            return UNBLENDING_INVALID_ARGUMENT;
         const Backend backend = select_backend(options->backend);
         if (backend == Backend::cpu) return UNBLENDING_INVALID_ARGUMENT;
         if (backend == Backend::cuda) return UNBLENDING_DEVICE_ERROR;
@@ -304,7 +298,7 @@ int32_t batch_call(const unblending_hip_image* images, size_t count, const unble
             if (im.layer_count > size_t(kMaxLayers))
                 throw ExecutionError("Layer count is outside the device solver's domain (1 to 8 layers)");
         }
-        if (!workspace || workspace_size < batch_workspace_bytes(images, count, refine, smooth))
+        if (!workspace || workspace_size < batch_workspace_bytes(images, count, refine, smooth, p))
             return UNBLENDING_BUFFER_TOO_SMALL;
         run_batch(images, sets, refine, opaque, smooth, p, workspace, static_cast<hipStream_t>(stream), stats);
         return UNBLENDING_OK;
@@ -370,9 +364,17 @@ UNBLENDING_API int32_t unblending_hip_refine_ex(const unblending_rgb_view* image
 UNBLENDING_API size_t unblending_hip_workspace_size(int32_t width, int32_t height, size_t layer_count,
                                                    int32_t refine, int32_t smooth_background, int32_t warm_start)
 {
-    if (width <= 0 || height <= 0 || layer_count == 0) return 0;
-    return workspace_bytes(size_t(width) * size_t(height), layer_count, refine != 0, smooth_background != 0,
-                           warm_start != 0);
+    (void)warm_start;  // the warm start's prediction needs no workspace
+    return unblending_hip_workspace_size_ex(width, height, layer_count, refine, smooth_background, nullptr);
+}
+
+UNBLENDING_API size_t unblending_hip_workspace_size_ex(int32_t width, int32_t height, size_t layer_count,
+                                                      int32_t refine, int32_t smooth_background,
+                                                      const unblending_hip_params* params)
+{
+    unblending_hip_params p;
+    if (width <= 0 || height <= 0 || layer_count == 0 || !read_params(params, p) || p.lbfgs_memory < 0) return 0;
+    return workspace_bytes(size_t(width) * size_t(height), layer_count, refine != 0, smooth_background != 0, p);
 }
 
 UNBLENDING_API int32_t unblending_hip_unmix_device(const double* rgb, int32_t width, int32_t height,
@@ -402,10 +404,18 @@ UNBLENDING_API int32_t unblending_hip_refine_device(const double* rgb, int32_t w
 UNBLENDING_API size_t unblending_hip_batch_workspace_size(const unblending_hip_image* images, size_t count,
                                                          int32_t refine, int32_t smooth_background)
 {
-    if (!images || count == 0) return 0;
+    return unblending_hip_batch_workspace_size_ex(images, count, refine, smooth_background, nullptr);
+}
+
+UNBLENDING_API size_t unblending_hip_batch_workspace_size_ex(const unblending_hip_image* images, size_t count,
+                                                            int32_t refine, int32_t smooth_background,
+                                                            const unblending_hip_params* params)
+{
+    unblending_hip_params p;
+    if (!images || count == 0 || !read_params(params, p) || p.lbfgs_memory < 0) return 0;
     for (size_t i = 0; i < count; ++i)
         if (images[i].width <= 0 || images[i].height <= 0 || images[i].layer_count == 0) return 0;
-    return batch_workspace_bytes(images, count, refine != 0, refine != 0 && smooth_background != 0);
+    return batch_workspace_bytes(images, count, refine != 0, refine != 0 && smooth_background != 0, p);
 }
 
 UNBLENDING_API int32_t unblending_hip_unmix_batch_device(const unblending_hip_image* images, size_t count,

@@ -49,7 +49,18 @@ struct DeviceJob {
     unblending_hip_stats* stats = nullptr;
 };
 
-size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool warm);
+// (s, y) pairs the inner minimiser keeps for n variables: memory when > 0,
+// else NLopt's default vector storage for LD_LBFGS (the reference never sets
+// it): max(1310720 / n, 10), lowered to max_evals when max_evals is not above
+// that (NLopt src/algs/luksan/plis.c) -- 1000 at the reference's maxeval.
+int lbfgs_memory(int n, int max_evals, int memory);
+constexpr int kMaxEvals = 1000;  // unblending.cpp:249
+
+// Copies a caller's params (struct_size bytes of them, at least up to
+// warm_outer_cap; later fields keep their defaults).  False when too short.
+bool read_params(const unblending_hip_params* params, unblending_hip_params& p);
+
+size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, const unblending_hip_params& params);
 size_t mlp_train_bytes(int batch);
 mlp::TrainBuffers mlp_train_view(void* ws, int batch);
 size_t filter_workspace_bytes(size_t n, int channels);
@@ -59,7 +70,8 @@ void run_filter(const double* rgb, int w, int h, const Channel* in, double* cons
                 int rows = 0);
 
 // Batches of device-resident images (unblending_hip_*_batch_device).
-size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth);
+size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth,
+                             const unblending_hip_params& params);
 void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& sets, bool refine, bool opaque,
                bool smooth, const unblending_hip_params& params, void* workspace, hipStream_t stream,
                unblending_hip_stats* stats);

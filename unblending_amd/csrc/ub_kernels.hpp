@@ -11,9 +11,15 @@
 namespace ubhip {
 
 constexpr int kMaxLayers = 8;      // layer counts with a compiled solve kernel
-constexpr int kLbfgsMemory = 1;    // (s, y) pairs of the inner minimiser
+constexpr int kLbfgsMemory = 1;    // (s, y) pairs of the register-resident inner minimiser
 constexpr int kMaxPersistentBlocks = 2048;  // 64-lane blocks of the persistent solve
 constexpr long long kLaneStateDoubles = (long long)kMaxPersistentBlocks * 64 * 4 * kMaxLayers;
+// 64-lane blocks of a persistent solve that keeps its (s, y) history in the
+// workspace: one wave per SIMD on the 256 CUs of an MI355X.  Each lane owns
+// a ring of min(memory, max_evals) records of kHistRecord(L) doubles
+// (s, y, 1 / s.y and the two-loop multiplier a).
+constexpr int kHistBlocks = 1024;
+constexpr int kHistRecord(int L) { return 8 * L + 2; }
 
 // Solver constants, unblending/src/unblending.cpp:220-249, plus the inner
 // minimiser's own (line-search trials).
@@ -23,6 +29,7 @@ struct SolverParams {
     int max_ls;      // 20
     int alpha_fixed; // refinement alphas fixed by bounds instead of constraints
     int warm_max_count;  // warm-started attempt: outer cap before the pixel is solved from the default start
+    int memory;      // (s, y) pairs kept: 1 = the register-resident minimiser, else a history in the workspace
     double ftol, xtol, eps, gamma, beta, rho0;
 };
 
@@ -34,8 +41,9 @@ struct SolverParams {
 // (trips of its loop times 64).
 enum Counter : int {
     kEvals = 0, kIterations = 1, kOuter = 2, kPixels = 3, kQueue = 4, kFallback = 5, kQueue2 = 6, kSlots = 7,
-    kGroupQueue = 8,  // batches: the queue of the launch of the g-th layer count is kGroupQueue + g
-    kNumCounters = 16
+    kPairs = 8,        // (s, y) pairs the two-loop recursions went through
+    kGroupQueue = 16,  // batches: the queue of the launch of the g-th layer count is kGroupQueue + g
+    kNumCounters = 24
 };
 
 struct PixelArgs {
@@ -91,6 +99,11 @@ struct BatchArgs {
     unsigned long long* counters;
     int queue;
     int max_blocks;          // 0: as many as fit on the device
+    // sp.memory != 1: the (s, y) history, hist_blocks blocks of 64 lanes x
+    // hist_cap records of kHistRecord(L) doubles, [block][lane][record]
+    double* hist;
+    int hist_cap;
+    int hist_blocks;
 };
 
 constexpr int kDefaultChunk = 64;  // pixels a wave takes from the queue at once

@@ -10,6 +10,7 @@
 // the reference's solve_cuda does (pixel_pipeline.cu:63-105), on a stream of
 // its own, and returns after synchronising it.
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -47,11 +48,12 @@ DevLayer to_device(const unblending_layer& l)
 
 constexpr int kDefaultWarmOuterCap = 8;
 
-SolverParams default_solver(const unblending_hip_params& p)
+SolverParams default_solver(const unblending_hip_params& p, int layers)
 {
     SolverParams s;
     s.warm_max_count = p.warm_outer_cap > 0 ? p.warm_outer_cap : kDefaultWarmOuterCap;
-    s.max_evals = 1000;   // unblending.cpp:249
+    s.max_evals = kMaxEvals;  // unblending.cpp:249
+    s.memory = lbfgs_memory(4 * layers, s.max_evals, p.lbfgs_memory);
     s.max_count = 20;     // :246
     s.max_ls = 20;
     s.alpha_fixed = p.alpha_fixed ? 1 : 0;
@@ -119,7 +121,45 @@ void require_device()
     if (e != hipSuccess || devices == 0) throw ExecutionError("No HIP device is available");
 }
 
+// Records each lane's ring holds: every pair the minimiser keeps, which is
+// never more than an inner solve's evaluations.
+int history_cap(int memory) { return std::min(memory, kMaxEvals); }
+
+// Blocks a history region holds for a launch over `chunks` pixel chunks of
+// the default size (a launch never has more blocks than chunks; launches
+// with smaller chunks are capped at this many blocks).
+int history_blocks(long long chunks) { return int(std::max(1LL, std::min<long long>(kHistBlocks, chunks))); }
+
+size_t history_bytes(long long chunks, int layers, int memory)
+{
+    if (memory == 1) return 0;
+    return align256(size_t(history_blocks(chunks)) * 64 * size_t(history_cap(memory)) *
+                    size_t(kHistRecord(layers)) * sizeof(double));
+}
+
+long long default_chunks(size_t n) { return (long long)((n + kDefaultChunk - 1) / kDefaultChunk); }
+
 }  // namespace
+
+int lbfgs_memory(int n, int max_evals, int memory)
+{
+    if (memory > 0) return memory;
+    int m = 1310720 / (n > 0 ? n : 1);
+    if (m < 10) m = 10;
+    if (max_evals > 0 && max_evals <= m) m = max_evals;
+    return m;
+}
+
+bool read_params(const unblending_hip_params* params, unblending_hip_params& p)
+{
+    p = UNBLENDING_HIP_PARAMS_INIT;
+    if (!params) return true;
+    const size_t minimum = offsetof(unblending_hip_params, warm_outer_cap) + sizeof(int32_t);
+    if (params->struct_size < minimum) return false;
+    std::memcpy(&p, params, std::min<size_t>(params->struct_size, sizeof p));
+    p.struct_size = sizeof p;
+    return true;
+}
 
 // filter workspace: the bounds slots, then every channel's coefficient planes
 constexpr size_t kBoundsBytes = 256;
@@ -174,16 +214,19 @@ size_t mlp_train_bytes(int batch)
     return size_t(reinterpret_cast<char*>(b.loss) - static_cast<char*>(nullptr)) + align256(mlp::kLossSlots * sizeof(float));
 }
 
-size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, bool warm)
+// Workspace of one pass: counters and descriptor, the per-lane outer
+// iterates, (refinement) the filter's coefficient and output planes, then
+// the (s, y) history.  The warm start's prediction needs none.
+size_t workspace_bytes(size_t n, size_t layers, bool refine, bool smooth, const unblending_hip_params& params)
 {
-    (void)warm;  // the warm start's prediction needs no workspace
     size_t bytes = kHeader + align256(size_t(kLaneStateDoubles) * sizeof(double));
     if (refine) {
         const int C = filter_channels(layers, smooth);
         bytes += filter_workspace_bytes(n, C);
         bytes += align256(size_t(C) * n * sizeof(double));
     }
-    return bytes;
+    const int M = lbfgs_memory(int(4 * layers), kMaxEvals, params.lbfgs_memory);
+    return bytes + history_bytes(default_chunks(n), int(layers), M);
 }
 
 void run_filter(const double* rgb, int w, int h, const Channel* in, double* const* q, int channels, int radius,
@@ -266,7 +309,7 @@ void run_device(const DeviceJob& job)
     for (size_t l = 0; l < L; ++l) args.layers[l] = to_device(job.set->host[l]);
     args.rgb = job.rgb;
     args.n = (long long)n;
-    args.sp = default_solver(job.params);
+    args.sp = default_solver(job.params, int(L));
     args.opaque = job.opaque ? 1 : 0;
     args.smooth = job.smooth ? 1 : 0;
     args.all_plus = job.set->all_plus ? 1 : 0;
@@ -315,6 +358,8 @@ void run_device(const DeviceJob& job)
     }
 
     if (warm && args.solver != 0) throw std::invalid_argument("The warm start runs on the persistent solve kernel");
+    if (args.sp.memory != 1 && args.solver != 0)
+        throw std::invalid_argument("The one-pixel-per-lane kernel keeps one (s, y) pair (lbfgs_memory = 1)");
     ImageDesc desc{};
     BatchArgs batch{};
     ImageDesc* ddesc = reinterpret_cast<ImageDesc*>(ws + kDescOffset);
@@ -342,6 +387,12 @@ void run_device(const DeviceJob& job)
         batch.lane_state = args.lane_state;
         batch.counters = counters;
         batch.queue = kQueue;
+        if (args.sp.memory != 1) {
+            const size_t hbytes = history_bytes(default_chunks(n), int(L), args.sp.memory);
+            batch.hist = reinterpret_cast<double*>(ws + workspace_bytes(n, L, refine, job.smooth, job.params) - hbytes);
+            batch.hist_cap = history_cap(args.sp.memory);
+            batch.hist_blocks = history_blocks(default_chunks(n));
+        }
     }
 
     Events es(timing);
@@ -360,6 +411,7 @@ void run_device(const DeviceJob& job)
         job.stats->pixels = host_counters[kPixels];
         job.stats->fallback_pixels = host_counters[kFallback];
         job.stats->lane_slots = host_counters[kSlots];
+        job.stats->history_pairs = host_counters[kPairs];
         job.stats->solve_ms = es.ms();
         job.stats->predict_ms = predict_ms;
     }
@@ -372,11 +424,13 @@ namespace {
 // one after another) and every image's filtered planes (image i's start
 // q_offset[i] doubles into the planes area).
 struct BatchLayout {
-    size_t desc, lane_state, ab, planes, total;
+    size_t desc, lane_state, ab, planes, hist, total;
     std::vector<size_t> q_offset;
+    int hist_blocks = 0, hist_cap = 0;
 };
 
-BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool refine, bool smooth)
+BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool refine, bool smooth,
+                         const unblending_hip_params& params)
 {
     BatchLayout b;
     b.desc = kDescOffset;
@@ -394,15 +448,29 @@ BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool 
         }
     }
     b.planes = b.ab + align256(ab_bytes);
-    b.total = b.planes + align256(planes * sizeof(double));
+    b.hist = b.planes + align256(planes * sizeof(double));
+    // one history region, shared by the launches of the batch (one after
+    // another on the stream), sized for the largest layer count's records
+    long long chunks = 0;
+    size_t hist = 0;
+    for (size_t i = 0; i < count; ++i) chunks += default_chunks(size_t(images[i].width) * size_t(images[i].height));
+    for (size_t i = 0; i < count; ++i) {
+        const int L = int(images[i].layer_count);
+        const int M = lbfgs_memory(4 * L, kMaxEvals, params.lbfgs_memory);
+        hist = std::max(hist, history_bytes(chunks, L, M));
+        if (M != 1) b.hist_cap = std::max(b.hist_cap, history_cap(M));
+    }
+    b.hist_blocks = history_blocks(chunks);
+    b.total = b.hist + hist;
     return b;
 }
 
 }  // namespace
 
-size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth)
+size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth,
+                             const unblending_hip_params& params)
 {
-    return batch_layout(images, count, refine, smooth).total;
+    return batch_layout(images, count, refine, smooth, params).total;
 }
 
 void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& sets, bool refine, bool opaque,
@@ -411,7 +479,7 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
 {
     const size_t count = sets.size();
     const bool timing = params.timing != 0;
-    const BatchLayout lay = batch_layout(images, count, refine, smooth);
+    const BatchLayout lay = batch_layout(images, count, refine, smooth, params);
     char* ws = static_cast<char*>(workspace);
     auto* counters = reinterpret_cast<unsigned long long*>(ws);
     ImageDesc* ddesc = reinterpret_cast<ImageDesc*>(ws + lay.desc);
@@ -459,7 +527,6 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return sets[a].host.size() < sets[b].host.size(); });
     const int chunk = params.chunk_size > 0 ? params.chunk_size : kDefaultChunk;
-    const SolverParams sp = default_solver(params);
     struct Group {
         int L;
         size_t first, count;
@@ -510,10 +577,15 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
         a.image_count = int(groups[g].count);
         a.chunk_size = chunk;
         a.chunk_count = groups[g].chunks;
-        a.sp = sp;
+        a.sp = default_solver(params, groups[g].L);
         a.lane_state = reinterpret_cast<double*>(ws + lay.lane_state);
         a.counters = counters;
         a.queue = kGroupQueue + int(g);
+        if (a.sp.memory != 1) {
+            a.hist = reinterpret_cast<double*>(ws + lay.hist);
+            a.hist_cap = history_cap(a.sp.memory);
+            a.hist_blocks = lay.hist_blocks;
+        }
         check(launch_batch_solve(groups[g].L, refine, a, stream), "batch solve");
     }
     es.stop(stream);
@@ -527,6 +599,7 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
         stats->outer = host_counters[kOuter];
         stats->pixels = host_counters[kPixels];
         stats->lane_slots = host_counters[kSlots];
+        stats->history_pairs = host_counters[kPairs];
         stats->solve_ms = es.ms();
     }
 }
@@ -542,8 +615,7 @@ void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const doubl
     DeviceBuffer d_rgb(3 * n * sizeof(double));
     DeviceBuffer d_out(4 * L * n * sizeof(double));
     DeviceBuffer d_first(first ? 4 * L * n * sizeof(double) : 0);
-    const bool warm = first == nullptr && params.warm_start != 0;
-    const size_t ws_bytes = workspace_bytes(n, L, first != nullptr, smooth, warm);
+    const size_t ws_bytes = workspace_bytes(n, L, first != nullptr, smooth, params);
     DeviceBuffer d_ws(ws_bytes);
     check(hipMemcpy2DAsync(d_rgb.p, 3 * w * sizeof(double), rgb.data, rgb.row_stride * sizeof(double),
                            3 * w * sizeof(double), h, hipMemcpyHostToDevice, stream.s),

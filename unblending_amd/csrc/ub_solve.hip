@@ -11,8 +11,13 @@
 // The layer set is wave-uniform and read with scalar loads (the simple kernel
 // from its argument segment, the persistent kernel from the image descriptor
 // through the constant address space).
-// The L-BFGS history is kept newest-last and shifted on each update so every
-// register index is static.
+// The one-pair minimiser keeps its (s, y) pair in registers; longer histories
+// (the default: NLopt's default vector storage keeps every pair of an inner
+// solve) live in per-lane rings in the workspace, and the wave forms the
+// search directions of its lanes cooperatively, 8 lanes a pixel (see
+// direction_service).  The two-loop recursion's dot products are summed in the
+// order orc_tree_dot of the oracle defines: eight partial sums of four
+// consecutive components, then pairwise.
 #include "ub_kernels.hpp"
 
 namespace ubhip {
@@ -28,6 +33,25 @@ __device__ __forceinline__ bool relstop(double vold, double vnew, double reltol)
 {
     return __builtin_fabs(vnew - vold) < reltol * (__builtin_fabs(vnew) + __builtin_fabs(vold)) * 0.5 ||
            vnew == vold;
+}
+
+// The two-loop recursion's dot product: partial sums of components
+// 4j .. 4j + 3 (j = 0..7), each in index order from 0.0, added pairwise.
+template <int N>
+__device__ __forceinline__ double tree_dot(const double* a, const double* b)
+{
+    static_assert(N <= 32, "four components a partial sum");
+    double p[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (4 * j + c < N) s = s + a[4 * j + c] * b[4 * j + c];
+        }
+        p[j] = s;
+    }
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
 }
 
 // Per-variable bounds (unblending.cpp:174-208): [0, 1]; alpha_0 = 1 when
@@ -112,10 +136,7 @@ __device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L
 #pragma unroll
         for (int k = M - 1; k >= 0; --k) {
             if (M - 1 - k < count) {
-                double s = 0.0;
-#pragma unroll
-                for (int i = 0; i < N; ++i) s += S[k][i] * q[i];
-                a[k] = rh[k] * s;
+                a[k] = rh[k] * tree_dot<N>(S[k], q);
 #pragma unroll
                 for (int i = 0; i < N; ++i) q[i] = q[i] - a[k] * Yh[k][i];
             }
@@ -125,10 +146,7 @@ __device__ void minimize(double* x, const Bounds<L, REFINE>& bd, const Problem<L
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             if (k >= M - count) {
-                double s = 0.0;
-#pragma unroll
-                for (int i = 0; i < N; ++i) s += Yh[k][i] * d[i];
-                const double b = rh[k] * s;
+                const double b = rh[k] * tree_dot<N>(Yh[k], d);
 #pragma unroll
                 for (int i = 0; i < N; ++i) d[i] = d[i] + (a[k] - b) * S[k][i];
             }
@@ -546,7 +564,7 @@ struct WaveQueue {
 // 1000 pixels of a launch) and the wave's trips around the evaluation loop
 // (wave-uniform).
 struct LaneCounters {
-    unsigned evals, iters, outer, pixels;
+    unsigned evals, iters, outer, pixels, pairs;
     unsigned long long trips;
 };
 
@@ -575,9 +593,113 @@ __device__ __forceinline__ bool claim_chunk(const BatchArgs& a, int& img, long l
     }
 }
 
+// Search directions from the (s, y) history, for the lanes in `pend`
+// (ballot of the lanes that need one; each has written its projected
+// gradient q to its LDS column dv[i * 64 + lane]).  The wave serves them 8
+// at a time: lane j of group g (lanes 8g .. 8g + 7) holds components
+// 4j .. 4j + 3 of the g-th pending pixel and streams that pixel's records,
+// newest to oldest and back, from its ring (hist + p * cap * R, the k-th
+// oldest record at slot (head + k) mod cap).  Dot products are the group's
+// four-component partial sums added pairwise over lanes (xor 1, 2, 4), which
+// every lane of the group forms identically.  The result H q replaces q in
+// dv.  The groups' trip counts are uniform, so the shuffles run on whole
+// groups.
+template <int N>
+__device__ __forceinline__ void direction_service(unsigned long long pend, double* dvcol, const double* hblock,
+                                                  int cap, int count, int head, double gamma)
+{
+    constexpr int R = 2 * N + 2;
+    const int lane = threadIdx.x, grp = lane >> 3, j = lane & 7;
+    // the records written at this trip's acceptances, by other lanes
+    __threadfence_block();
+    while (pend) {
+        unsigned long long m = pend;
+        for (int t = 0; t < grp; ++t) m &= m - 1ull;
+        const bool has = m != 0ull;
+        const int p = has ? __builtin_ctzll(m) : 0;
+        unsigned long long rest = pend;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) rest &= rest - 1ull;
+        pend = rest;
+        const int pc = __shfl(count, p);
+        const int hd = __shfl(head, p);
+        const double gam = __shfl(gamma, p);
+        const int cnt = has ? pc : 0;
+        int km = cnt;
+        km = max(km, __shfl_xor(km, 8));
+        km = max(km, __shfl_xor(km, 16));
+        km = max(km, __shfl_xor(km, 32));
+        km = __builtin_amdgcn_readfirstlane(km);
+        double* const ring = const_cast<double*>(hblock) + (long long)p * cap * R;
+        double qv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = 4 * j + c;
+            qv[c] = (has && i < N) ? dvcol[i * 64 + p] : 0.0;
+        }
+        for (int k = km - 1; k >= 0; --k) {
+            if (k < cnt) {
+                int slot = hd + k;
+                if (slot >= cap) slot -= cap;
+                double* const rec = ring + (long long)slot * R;
+                double part = 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int i = 4 * j + c;
+                    if (i < N) part = part + rec[i] * qv[c];
+                }
+                part = part + __shfl_xor(part, 1);
+                part = part + __shfl_xor(part, 2);
+                part = part + __shfl_xor(part, 4);
+                const double a = rec[2 * N] * part;
+                if (j == 0) rec[2 * N + 1] = a;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int i = 4 * j + c;
+                    if (i < N) qv[c] = qv[c] - a * rec[N + i];
+                }
+            }
+        }
+        // the multipliers a_k written above, read back below
+        __threadfence_block();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) qv[c] = gam * qv[c];
+        for (int k = 0; k < km; ++k) {
+            if (k < cnt) {
+                int slot = hd + k;
+                if (slot >= cap) slot -= cap;
+                const double* const rec = ring + (long long)slot * R;
+                double part = 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int i = 4 * j + c;
+                    if (i < N) part = part + rec[N + i] * qv[c];
+                }
+                part = part + __shfl_xor(part, 1);
+                part = part + __shfl_xor(part, 2);
+                part = part + __shfl_xor(part, 4);
+                const double b = rec[2 * N] * part;
+                const double a = rec[2 * N + 1];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int i = 4 * j + c;
+                    if (i < N) qv[c] = qv[c] + (a - b) * rec[i];
+                }
+            }
+        }
+        if (has) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int i = 4 * j + c;
+                if (i < N) dvcol[i * 64 + p] = qv[c];
+            }
+        }
+    }
+}
+
 // The solve of the wave's current image and of the images that follow it in
 // the queue (all with L layers).  Returns when the queue is exhausted.
-template <int L, bool REFINE>
+template <int L, bool REFINE, bool HIST>
 __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, WaveQueue& q, LaneCounters& lc)
 {
     using PS = PersistentShape<L, REFINE>;
@@ -603,6 +725,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     double rh = 0.0;
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
     int phase = kPhNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
+    int head = 0;  // history: slot of the oldest record of the ring
     bool warm_try = false;  // this pixel's solve started from the warm-start prediction
     long long pix = 0;
 #pragma unroll
@@ -716,6 +839,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             constraints<L, REFINE>(x0, lay, tg0, ta0, C0);
             cn_outer = norm<NC>(C0);
             count = 0;
+            head = 0;
             gamma = 1.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
@@ -802,9 +926,35 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 if (sy > 2.220446049250313e-16 * yy) {
                     rh = 1.0 / sy;
                     gamma = sy / yy;
-                    count = 1;
+                    if constexpr (HIST) {
+                        // the pair goes to the ring: appended, or over the
+                        // oldest record once the ring is full
+                        const int cap = (&args.hist_cap)[zero];
+                        int slot;
+                        if (count < cap) {
+                            slot = head + count;
+                            if (slot >= cap) slot -= cap;
+                            ++count;
+                        } else {
+                            slot = head;
+                            head = (head + 1 == cap) ? 0 : head + 1;
+                        }
+                        constexpr int R = 2 * N + 2;
+                        double* const rec = (&args.hist)[zero] +
+                                            ((long long)blockIdx.x * 64 + lane) * (long long)cap * R +
+                                            (long long)slot * R;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) {
+                            rec[i] = S[i];
+                            rec[N + i] = Y[i];
+                        }
+                        rec[2 * N] = rh;
+                    } else {
+                        count = 1;
+                    }
                 } else {
                     count = 0;
+                    head = 0;
                     gamma = 1.0;
                 }
                 if (nev >= sp.max_evals || relstop(fold, f, sp.ftol) || xstop) inner_end = true;
@@ -827,10 +977,11 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             }
         }
 
+        unsigned int freev = 0;
+        bool need_dir = false;  // history: the direction is formed by the wave below
         if (direction) {
             // free set and projected gradient; q is then turned into d in place
             bool finite = __builtin_isfinite(f);
-            unsigned int freev = 0;
             double q[N];
             double pgmax = 0.0;
 #pragma unroll
@@ -847,24 +998,25 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             }
             if (!finite || pgmax == 0.0) {
                 inner_end = true;
+            } else if constexpr (HIST) {
+                ++lc.iters;
+                lc.pairs += (unsigned)count;
+#pragma unroll
+                for (int i = 0; i < N; ++i) dv[i] = q[i];
+                need_dir = true;
             } else {
                 ++lc.iters;
+                lc.pairs += (unsigned)count;
                 double a = 0.0;
                 if (count > 0) {
-                    double sd = 0.0;
-#pragma unroll
-                    for (int i = 0; i < N; ++i) sd += S[i] * q[i];
-                    a = rh * sd;
+                    a = rh * tree_dot<N>(S, q);
 #pragma unroll
                     for (int i = 0; i < N; ++i) q[i] = q[i] - a * Y[i];
                 }
 #pragma unroll
                 for (int i = 0; i < N; ++i) q[i] = gamma * q[i];
                 if (count > 0) {
-                    double sd = 0.0;
-#pragma unroll
-                    for (int i = 0; i < N; ++i) sd += Y[i] * q[i];
-                    const double b = rh * sd;
+                    const double b = rh * tree_dot<N>(Y, q);
 #pragma unroll
                     for (int i = 0; i < N; ++i) q[i] = q[i] + (a - b) * S[i];
                 }
@@ -891,6 +1043,43 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 ls = 0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) dv[i] = q[i];
+                phase = kPhTrial;
+            }
+        }
+        if constexpr (HIST) {
+            const unsigned long long pend = __ballot(need_dir);
+            if (pend) {
+                constexpr int R = 2 * N + 2;
+                const int cap = (&args.hist_cap)[zero];
+                const double* hblock = (&args.hist)[zero] + (long long)blockIdx.x * 64 * (long long)cap * R;
+                direction_service<N>(pend, smem + N * 64, hblock, cap, count, head, gamma);
+            }
+            if (need_dir) {
+                double qd[N];
+                double gd = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    qd[i] = ((freev >> i) & 1u) ? -dv[i] : 0.0;
+                    gd += gv[i] * qd[i];
+                }
+                if (!(gd < 0.0)) {
+                    count = 0;
+                    head = 0;
+                    gamma = 1.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) qd[i] = ((freev >> i) & 1u) ? -gv[i] : 0.0;
+                }
+                t = 1.0;
+                if (count == 0) {
+                    double dd = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) dd += qd[i] * qd[i];
+                    const double inv = 1.0 / __builtin_sqrt(dd);
+                    t = (inv < 1.0) ? inv : 1.0;
+                }
+                ls = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) dv[i] = qd[i];
                 phase = kPhTrial;
             }
         }
@@ -947,6 +1136,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             } else {
                 ++ocount;
                 count = 0;
+                head = 0;
                 gamma = 1.0;
                 phase = kPhStart;
             }
@@ -954,11 +1144,11 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     }
 }
 
-template <int L, bool REFINE>
+template <int L, bool REFINE, bool HIST>
 __global__ void __launch_bounds__(64) solve_batch_kernel(BatchArgs args)
 {
     __shared__ double smem[persistent_lds_doubles<L, REFINE>() * 64];
-    LaneCounters lc{0, 0, 0, 0, 0};
+    LaneCounters lc{0, 0, 0, 0, 0, 0};
     WaveQueue q{};
     q.img = -1;
     int i0;
@@ -969,27 +1159,29 @@ __global__ void __launch_bounds__(64) solve_batch_kernel(BatchArgs args)
         q.end = e0;
     }
     // every image of the launch has L layers (the host groups them so)
-    if (q.img >= 0) run_images<L, REFINE>(args, smem, q, lc);
-    unsigned long long evals = lc.evals, iters = lc.iters, outer = lc.outer, pixels = lc.pixels;
+    if (q.img >= 0) run_images<L, REFINE, HIST>(args, smem, q, lc);
+    unsigned long long evals = lc.evals, iters = lc.iters, outer = lc.outer, pixels = lc.pixels, pairs = lc.pairs;
     for (int off = 32; off > 0; off >>= 1) {
         evals += __shfl_xor(evals, off);
         iters += __shfl_xor(iters, off);
         outer += __shfl_xor(outer, off);
         pixels += __shfl_xor(pixels, off);
+        pairs += __shfl_xor(pairs, off);
     }
     if (threadIdx.x == 0) {
         atomicAdd(&args.counters[kEvals], evals);
         atomicAdd(&args.counters[kIterations], iters);
         atomicAdd(&args.counters[kOuter], outer);
         atomicAdd(&args.counters[kPixels], pixels);
+        atomicAdd(&args.counters[kPairs], pairs);
         atomicAdd(&args.counters[kSlots], 64ull * lc.trips);
     }
 }
 
-template <int L, bool REFINE>
+template <int L, bool REFINE, bool HIST>
 hipError_t launch_batch(const BatchArgs& args, hipStream_t stream)
 {
-    auto kernel = solve_batch_kernel<L, REFINE>;
+    auto kernel = solve_batch_kernel<L, REFINE, HIST>;
     int device = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
@@ -1000,6 +1192,8 @@ hipError_t launch_batch(const BatchArgs& args, hipStream_t stream)
     if (blocks > args.chunk_count) blocks = args.chunk_count;
     if (args.max_blocks > 0 && blocks > args.max_blocks) blocks = args.max_blocks;
     if (blocks > kMaxPersistentBlocks) blocks = kMaxPersistentBlocks;
+    // a history-keeping launch has ring storage for hist_blocks blocks
+    if (HIST && blocks > args.hist_blocks) blocks = args.hist_blocks;
     if (blocks < 1) return hipSuccess;
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(64), 0, stream, args);
     return hipGetLastError();
@@ -1051,7 +1245,11 @@ hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream)
 template <int L>
 hipError_t launch_batch_L(bool refine, const BatchArgs& args, hipStream_t stream)
 {
-    return refine ? launch_batch<L, true>(args, stream) : launch_batch<L, false>(args, stream);
+    if (args.sp.memory != 1) {
+        if (!args.hist || args.hist_cap < 1 || args.hist_blocks < 1) return hipErrorInvalidValue;
+        return refine ? launch_batch<L, true, true>(args, stream) : launch_batch<L, false, true>(args, stream);
+    }
+    return refine ? launch_batch<L, true, false>(args, stream) : launch_batch<L, false, false>(args, stream);
 }
 
 hipError_t launch_batch_solve(int L, bool refine, const BatchArgs& args, hipStream_t stream)

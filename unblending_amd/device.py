@@ -35,15 +35,27 @@ def _stream_ptr(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def workspace_size(w, h, L, refine, smooth=False, warm=False) -> int:
-    return int(_lib.unblending_hip_workspace_size(int(w), int(h), int(L), int(bool(refine)), int(bool(smooth)),
-                                                  int(bool(warm))))
-
-
-def workspace(w, h, L, refine, smooth=False, device=None, warm=False):
+def _empty(shape, device, stream):
+    """A float64 tensor for work enqueued on `stream`: allocated on that
+    stream, so torch's caching allocator does not hand its memory to another
+    stream's allocation while the library's kernels still use it."""
     torch = _torch()
-    n = max(1, (workspace_size(w, h, L, refine, smooth, warm) + 7) // 8)
-    return torch.empty(n, dtype=torch.float64, device=device or "cuda")
+    if stream is None:
+        return torch.empty(shape, dtype=torch.float64, device=device)
+    with torch.cuda.stream(stream):
+        return torch.empty(shape, dtype=torch.float64, device=device)
+
+
+def workspace_size(w, h, L, refine, smooth=False, warm=False, params: HipParams | None = None) -> int:
+    """Workspace bytes for the device calls with these params (None: defaults)."""
+    del warm  # the warm start's prediction needs no workspace
+    return int(_lib.unblending_hip_workspace_size_ex(int(w), int(h), int(L), int(bool(refine)), int(bool(smooth)),
+                                                     ctypes.byref(params) if params is not None else None))
+
+
+def workspace(w, h, L, refine, smooth=False, device=None, warm=False, params: HipParams | None = None, stream=None):
+    n = max(1, (workspace_size(w, h, L, refine, smooth, warm, params) + 7) // 8)
+    return _empty(n, device or "cuda", stream)
 
 
 def unmix(rgb, layer_infos, opaque=True, out=None, alpha_planes=None, ws=None, params: HipParams | None = None,
@@ -53,12 +65,12 @@ def unmix(rgb, layer_infos, opaque=True, out=None, alpha_planes=None, ws=None, p
     L = len(layer_infos)
     _check(rgb, (h, w, 3), "rgb")
     if out is None:
-        out = torch.empty((L, h, w, 4), dtype=torch.float64, device=rgb.device)
+        out = _empty((L, h, w, 4), rgb.device, stream)
     _check(out, (L, h, w, 4), "out")
     if alpha_planes is not None:
         _check(alpha_planes, (L, h, w), "alpha_planes")
     if ws is None:
-        ws = workspace(w, h, L, False, device=rgb.device, warm=bool(params is not None and params.warm_start))
+        ws = workspace(w, h, L, False, device=rgb.device, params=params, stream=stream)
     opt = options(BACKEND_HIP, opaque, False)
     st = _lib.unblending_hip_unmix_device(
         ctypes.c_void_p(rgb.data_ptr()), w, h, layer_array(layer_infos), L, ctypes.byref(opt),
@@ -80,10 +92,10 @@ def refine(rgb, first, layer_infos, opaque=True, smooth=False, alpha_planes=None
     if alpha_planes is not None:
         _check(alpha_planes, (L, h, w), "alpha_planes")
     if out is None:
-        out = torch.empty((L, h, w, 4), dtype=torch.float64, device=rgb.device)
+        out = _empty((L, h, w, 4), rgb.device, stream)
     _check(out, (L, h, w, 4), "out")
     if ws is None:
-        ws = workspace(w, h, L, True, smooth, device=rgb.device)
+        ws = workspace(w, h, L, True, smooth, device=rgb.device, params=params, stream=stream)
     opt = options(BACKEND_HIP, opaque, smooth)
     st = _lib.unblending_hip_refine_device(
         ctypes.c_void_p(rgb.data_ptr()), w, h, layer_array(layer_infos), L,
@@ -108,10 +120,10 @@ def guided_filter(rgb, planes, radius, eps, out=None, ws=None, stream=None, stat
     for i, p in enumerate(planes):
         _check(p, (h, w), "planes[%d]" % i)
     if out is None:
-        out = [torch.empty((h, w), dtype=torch.float64, device=rgb.device) for _ in range(C)]
+        out = [_empty((h, w), rgb.device, stream) for _ in range(C)]
     if ws is None:
         n = max(1, (int(_lib.unblending_hip_filter_workspace_size(w, h, C)) + 7) // 8)
-        ws = torch.empty(n, dtype=torch.float64, device=rgb.device)
+        ws = _empty(n, rgb.device, stream)
     pin = (ctypes.c_void_p * C)(*[p.data_ptr() for p in planes])
     qout = (ctypes.c_void_p * C)(*[q.data_ptr() for q in out])
     st = _lib.unblending_hip_guided_filter_device(
@@ -129,7 +141,7 @@ def composite(layers, layer_infos, out=None, stream=None):
     L, h, w, _ = layers.shape
     _check(layers, (L, h, w, 4), "layers")
     if out is None:
-        out = torch.empty((h, w, 4), dtype=torch.float64, device=layers.device)
+        out = _empty((h, w, 4), layers.device, stream)
     _check(out, (h, w, 4), "out")
     ops, md = _composite_tables([i.comp_op for i in layer_infos], [i.blend_mode for i in layer_infos])
     raise_for_status(_lib.unblending_hip_composite_device(ctypes.c_void_p(layers.data_ptr()), w, h, L, ops, md,
@@ -179,22 +191,22 @@ def _batch_array(images):
     return arr
 
 
-def batch_workspace_size(images, refine, smooth=False) -> int:
-    return int(_lib.unblending_hip_batch_workspace_size(_batch_array(images), len(images), int(bool(refine)),
-                                                        int(bool(smooth))))
+def batch_workspace_size(images, refine, smooth=False, params: HipParams | None = None) -> int:
+    return int(_lib.unblending_hip_batch_workspace_size_ex(_batch_array(images), len(images), int(bool(refine)),
+                                                           int(bool(smooth)),
+                                                           ctypes.byref(params) if params is not None else None))
 
 
-def batch_workspace(images, refine, smooth=False, device=None):
-    torch = _torch()
-    n = max(1, (batch_workspace_size(images, refine, smooth) + 7) // 8)
-    return torch.empty(n, dtype=torch.float64, device=device or images[0].rgb.device)
+def batch_workspace(images, refine, smooth=False, device=None, params: HipParams | None = None, stream=None):
+    n = max(1, (batch_workspace_size(images, refine, smooth, params) + 7) // 8)
+    return _empty(n, device or images[0].rgb.device, stream)
 
 
 def unmix_batch(images, opaque=True, ws=None, params: HipParams | None = None, stream=None,
                 stats: HipStats | None = None):
     """First pass of every BatchImage (outputs, and alpha planes when given)."""
     if ws is None:
-        ws = batch_workspace(images, False)
+        ws = batch_workspace(images, False, params=params, stream=stream)
     opt = options(BACKEND_HIP, opaque, False)
     st = _lib.unblending_hip_unmix_batch_device(
         _batch_array(images), len(images), ctypes.byref(opt), ctypes.byref(params or hip_params()),
@@ -208,7 +220,7 @@ def refine_batch(images, opaque=True, smooth=False, ws=None, params: HipParams |
                  stats: HipStats | None = None):
     """Refinement of every BatchImage (each with its first pass)."""
     if ws is None:
-        ws = batch_workspace(images, True, smooth)
+        ws = batch_workspace(images, True, smooth, params=params, stream=stream)
     opt = options(BACKEND_HIP, opaque, smooth)
     st = _lib.unblending_hip_refine_batch_device(
         _batch_array(images), len(images), ctypes.byref(opt), ctypes.byref(params or hip_params()),
