@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=640)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solver", type=int, default=0)
+    ap.add_argument("--lbfgs-memory", type=int, default=0,
+                    help="(s, y) pairs of the inner minimiser: 0 = NLopt's default (every pair), 1 = fast mode")
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-size", type=int, default=512)
@@ -276,8 +278,9 @@ def main():
     first = torch.empty((L, h, w, 4), dtype=torch.float64, device=dev)
     out = torch.empty_like(first)
     alpha = torch.empty((L, h, w), dtype=torch.float64, device=dev)
-    ws1 = device.workspace(w, h, L, False, device=dev)
-    ws2 = device.workspace(w, h, L, True, device=dev)
+    mem = {"lbfgs_memory": args.lbfgs_memory}
+    ws1 = device.workspace(w, h, L, False, device=dev, params=U.hip_params(**mem))
+    ws2 = device.workspace(w, h, L, True, device=dev, params=U.hip_params(**mem))
     t0 = time.perf_counter()
     if args.no_tune:
         shape = {"chunk_size": 0, "filter_block": 0, "filter_rows": 0}
@@ -286,7 +289,7 @@ def main():
         chunk, _ = autotune.tune_chunk(rgb, infos, log=log)
         shape = {"chunk_size": chunk, "filter_block": block, "filter_rows": rows}
     tune_s = time.perf_counter() - t0
-    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape)
+    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape, **mem)
 
     def step(p=params, s1=None, s2=None, o=out):
         device.unmix(rgb, infos, True, out=first, alpha_planes=alpha, ws=ws1, params=p, stats=s1)
@@ -316,7 +319,7 @@ def main():
         return s1, s2
 
     ms_plain = timed(params, out, "plain")
-    s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape), out)
+    s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape, **mem), out)
     first_plain = first.clone()
 
     def recon(packed):
@@ -426,6 +429,7 @@ def main():
                   "unmix_evals_per_px": s1.evals / n, "unmix_iters_per_px": s1.iterations / n,
                   "unmix_outer_per_px": s1.outer / n, "refine_evals_per_px": s2.evals / n,
                   "refine_iters_per_px": s2.iterations / n, "refine_outer_per_px": s2.outer / n,
+                  "unmix_pairs_per_px": s1.history_pairs / n, "refine_pairs_per_px": s2.history_pairs / n,
                   "unmix_lane_use": s1.evals / s1.lane_slots if s1.lane_slots else None,
                   "refine_lane_use": s2.evals / s2.lane_slots if s2.lane_slots else None,
                   "unmix_tflops": round(flops1 / (s1.solve_ms * 1e-3) / 1e12, 3),

@@ -16,10 +16,12 @@ constexpr int kMaxPersistentBlocks = 2048;  // 64-lane blocks of the persistent 
 constexpr long long kLaneStateDoubles = (long long)kMaxPersistentBlocks * 64 * 4 * kMaxLayers;
 // 64-lane blocks of a persistent solve that keeps its (s, y) history in the
 // workspace: one wave per SIMD on the 256 CUs of an MI355X.  Each lane owns
-// a ring of min(memory, max_evals) records of kHistRecord(L) doubles
-// (s, y, 1 / s.y and the two-loop multiplier a).
+// a ring of min(memory, max_evals) records of kHistRecord(L) doubles: s and
+// y, each zero-padded from 4L to kHistPad(L) (a multiple of 8) components,
+// then 1 / s.y and the two-loop multiplier a.
 constexpr int kHistBlocks = 1024;
-constexpr int kHistRecord(int L) { return 8 * L + 2; }
+constexpr int kHistPad(int L) { return 8 * ((L + 1) / 2); }
+constexpr int kHistRecord(int L) { return 2 * kHistPad(L) + 2; }
 
 // Solver constants, unblending/src/unblending.cpp:220-249, plus the inner
 // minimiser's own (line-search trials).

@@ -485,7 +485,10 @@ __global__ void __launch_bounds__(64) solve_simple_kernel(PixelArgs args)
 // per-lane global slot.
 // ---------------------------------------------------------------------------
 // per-lane phases of the persistent solve (distinct from the Counter slots)
-enum Phase : int { kPhNeedPixel = 0, kPhStart = 1, kPhTrial = 2, kPhWait = 3, kPhOuter = 4, kPhRetry = 5 };
+enum Phase : int {
+    kPhNeedPixel = 0, kPhStart = 1, kPhTrial = 2, kPhWait = 3, kPhOuter = 4, kPhRetry = 5,
+    kPhDir = 6  // history: search direction in progress (direction_service)
+};
 
 // Lanes waiting for a pixel, or for their outer update, are served together
 // once at least this many wait (or nothing else is left to do).
@@ -593,108 +596,175 @@ __device__ __forceinline__ bool claim_chunk(const BatchArgs& a, int& img, long l
     }
 }
 
-// Search directions from the (s, y) history, for the lanes in `pend`
-// (ballot of the lanes that need one; each has written its projected
-// gradient q to its LDS column dv[i * 64 + lane]).  The wave serves them 8
-// at a time: lane j of group g (lanes 8g .. 8g + 7) holds components
-// 4j .. 4j + 3 of the g-th pending pixel and streams that pixel's records,
-// newest to oldest and back, from its ring (hist + p * cap * R, the k-th
-// oldest record at slot (head + k) mod cap).  Dot products are the group's
-// four-component partial sums added pairwise over lanes (xor 1, 2, 4), which
-// every lane of the group forms identically.  The result H q replaces q in
-// dv.  The groups' trip counts are uniform, so the shuffles run on whole
-// groups.
-template <int N>
-__device__ __forceinline__ void direction_service(unsigned long long pend, double* dvcol, const double* hblock,
-                                                  int cap, int count, int head, double gamma)
+// A double moved between lanes by a DPP pattern (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
 {
-    constexpr int R = 2 * N + 2;
-    const int lane = threadIdx.x, grp = lane >> 3, j = lane & 7;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over a 4-lane group (a quad) of the lanes' values A_l, where lane l's
+// A_l = p_2l + p_2l+1: quad_perm [1,0,3,2] then [2,3,0,1], giving
+// ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)) -- the tree order of the
+// oracle's orc_tree_dot.  Every lane of the quad ends with the same value
+// (each addition's operands are the same pair, in either order).
+__device__ __forceinline__ double quad_sum(double v)
+{
+    v = v + dpp_f64<0xB1>(v);
+    v = v + dpp_f64<0x4E>(v);
+    return v;
+}
+
+// Records of one history ring streamed per load batch of the two-loop
+// recursion, and the recursion steps a pending direction advances per trip.
+constexpr int kHistBlock = 3;
+constexpr int kDirBudget = 32;
+
+// Search directions from the (s, y) history for the lanes in `pend` (those
+// whose direction is pending).  A direction of a pixel with `count` records
+// is 2 count steps of the two-loop recursion -- steps 0 .. count - 1 newest
+// to oldest (a_k = rho_k s_k.q, q -= a_k y_k), then q = gamma q, steps
+// count .. 2 count - 1 oldest to newest (b = rho_k y_k.q, q += (a_k - b) s_k);
+// `dstep` counts the steps done, and the vector being transformed stays in
+// the pixel's LDS column dv[i * 64 + lane] between trips.  Each call advances
+// every pending direction by at most kDirBudget steps, so that a pixel with a
+// long history does not hold up the evaluations of the rest of the wave.
+// The wave serves the pending pixels 16 at a time, longest remaining work
+// first: lane l of quad g (lanes 4g .. 4g + 3) holds components 8l .. 8l + 7
+// of one pending pixel (the records' zero padding makes every component of a
+// record present) and streams that pixel's records -- the k-th oldest at slot
+// (head + k) mod cap of its ring hist + p * cap * R -- kHistBlock records a
+// load batch.  A dot product is each lane's two four-component partial sums,
+// added, then summed over the quad (quad_sum).  A quad's trip counts are
+// uniform over the quad, so its DPP moves run on whole quads.  Returns the
+// lanes whose direction is complete.
+template <int L>
+__device__ __forceinline__ unsigned long long direction_service(unsigned long long pend, double* dvcol,
+                                                                const double* hblock, int cap, int count,
+                                                                int head, double gamma, int& dstep)
+{
+    constexpr int N = 4 * L;
+    constexpr int NP = kHistPad(L);
+    constexpr int R = kHistRecord(L);
+    constexpr int U = kHistBlock;
+    const int lane = threadIdx.x, grp = lane >> 2, l = lane & 3;
+    const bool live = 8 * l < NP;
+    const bool mine = (pend >> lane) & 1ull;
+    // rank among the pending lanes: most remaining steps first
+    const int rem = 2 * count - dstep;
+    int rank = 0;
+    {
+        unsigned long long m = pend;
+        while (m) {
+            const int o = __builtin_ctzll(m);
+            m &= m - 1ull;
+            const int ro = __builtin_amdgcn_readlane(rem, o);
+            rank += (ro > rem || (ro == rem && o < lane)) ? 1 : 0;
+        }
+    }
     // the records written at this trip's acceptances, by other lanes
     __threadfence_block();
-    while (pend) {
-        unsigned long long m = pend;
+    const int rounds = (__popcll(pend) + 15) >> 4;
+    for (int r = 0; r < rounds; ++r) {
+        const unsigned long long rmask = __ballot(mine && (rank >> 4) == r);
+        unsigned long long m = rmask;
         for (int t = 0; t < grp; ++t) m &= m - 1ull;
         const bool has = m != 0ull;
         const int p = has ? __builtin_ctzll(m) : 0;
-        unsigned long long rest = pend;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) rest &= rest - 1ull;
-        pend = rest;
         const int pc = __shfl(count, p);
         const int hd = __shfl(head, p);
         const double gam = __shfl(gamma, p);
+        const int pd = __shfl(dstep, p);
         const int cnt = has ? pc : 0;
-        int km = cnt;
-        km = max(km, __shfl_xor(km, 8));
-        km = max(km, __shfl_xor(km, 16));
-        km = max(km, __shfl_xor(km, 32));
-        km = __builtin_amdgcn_readfirstlane(km);
+        int pos = has ? pd : 0;
+        const int end = has ? min(2 * cnt, pos + kDirBudget) : 0;
         double* const ring = const_cast<double*>(hblock) + (long long)p * cap * R;
-        double qv[4];
+        double qv[8];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int i = 4 * j + c;
-            qv[c] = (has && i < N) ? dvcol[i * 64 + p] : 0.0;
+        for (int c = 0; c < 8; ++c) qv[c] = (has && 8 * l + c < N) ? dvcol[(8 * l + c) * 64 + p] : 0.0;
+        if (has && cnt == 0) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
         }
-        for (int k = km - 1; k >= 0; --k) {
-            if (k < cnt) {
-                int slot = hd + k;
-                if (slot >= cap) slot -= cap;
-                double* const rec = ring + (long long)slot * R;
-                double part = 0.0;
+        while (__ballot(pos < end)) {
+            // a batch stays within one loop of the recursion (its loads of a_k
+            // follow the stores of the first loop in the wave's program order,
+            // which the wave's memory operations to one address keep)
+            const int lim = (pos < cnt && cnt < end) ? cnt : end;
+            const int take = min(U, lim - pos);
+            const bool back = pos >= cnt;  // second loop, oldest to newest
+            double sv[U][8], yv[U][8], rv[U], av[U];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int i = 4 * j + c;
-                    if (i < N) part = part + rec[i] * qv[c];
-                }
-                part = part + __shfl_xor(part, 1);
-                part = part + __shfl_xor(part, 2);
-                part = part + __shfl_xor(part, 4);
-                const double a = rec[2 * N] * part;
-                if (j == 0) rec[2 * N + 1] = a;
+            for (int u = 0; u < U; ++u) {
+                if (u < take) {
+                    const int k = back ? pos + u - cnt : cnt - 1 - (pos + u);
+                    int slot = hd + k;
+                    if (slot >= cap) slot -= cap;
+                    const double* rec = ring + (long long)slot * R;
+                    rv[u] = rec[2 * NP];
+                    av[u] = rec[2 * NP + 1];
+                    if (live) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int i = 4 * j + c;
-                    if (i < N) qv[c] = qv[c] - a * rec[N + i];
+                        for (int c = 0; c < 8; ++c) {
+                            sv[u][c] = rec[8 * l + c];
+                            yv[u][c] = rec[NP + 8 * l + c];
+                        }
+                    }
                 }
             }
-        }
-        // the multipliers a_k written above, read back below
-        __threadfence_block();
+            if (back && take > 0 && pos == cnt) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) qv[c] = gam * qv[c];
-        for (int k = 0; k < km; ++k) {
-            if (k < cnt) {
-                int slot = hd + k;
-                if (slot >= cap) slot -= cap;
-                const double* const rec = ring + (long long)slot * R;
-                double part = 0.0;
+                for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
+            }
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int i = 4 * j + c;
-                    if (i < N) part = part + rec[N + i] * qv[c];
-                }
-                part = part + __shfl_xor(part, 1);
-                part = part + __shfl_xor(part, 2);
-                part = part + __shfl_xor(part, 4);
-                const double b = rec[2 * N] * part;
-                const double a = rec[2 * N + 1];
+            for (int u = 0; u < U; ++u) {
+                if (u < take) {
+                    double p0 = 0.0, p1 = 0.0;
+                    if (live) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int i = 4 * j + c;
-                    if (i < N) qv[c] = qv[c] + (a - b) * rec[i];
+                        for (int c = 0; c < 4; ++c) p0 = p0 + (back ? yv[u][c] : sv[u][c]) * qv[c];
+#pragma unroll
+                        for (int c = 4; c < 8; ++c) p1 = p1 + (back ? yv[u][c] : sv[u][c]) * qv[c];
+                    }
+                    const double dot = quad_sum(p0 + p1);
+                    if (!back) {
+                        const double a = rv[u] * dot;
+                        if (l == 0) {
+                            int slot = hd + cnt - 1 - (pos + u);
+                            if (slot >= cap) slot -= cap;
+                            ring[(long long)slot * R + 2 * NP + 1] = a;
+                        }
+                        if (live) {
+#pragma unroll
+                            for (int c = 0; c < 8; ++c) qv[c] = qv[c] - a * yv[u][c];
+                        }
+                    } else {
+                        const double b = rv[u] * dot;
+                        if (live) {
+#pragma unroll
+                            for (int c = 0; c < 8; ++c) qv[c] = qv[c] + (av[u] - b) * sv[u][c];
+                        }
+                    }
                 }
             }
+            pos += take;
         }
         if (has) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int i = 4 * j + c;
-                if (i < N) dvcol[i * 64 + p] = qv[c];
+            for (int c = 0; c < 8; ++c) {
+                if (8 * l + c < N) dvcol[(8 * l + c) * 64 + p] = qv[c];
             }
         }
+        // the owner of each served pixel takes its new step count from the
+        // quad's first lane
+        const int own_g = __popcll(rmask & ((1ull << lane) - 1ull));
+        const int np = __shfl(pos, 4 * (own_g & 15));
+        if (mine && (rank >> 4) == r) dstep = np;
     }
+    return __ballot(mine && dstep == 2 * count);
 }
 
 // The solve of the wave's current image and of the images that follow it in
@@ -726,6 +796,8 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
     double f = 0.0, t = 1.0, rho = 0.0, gamma = 1.0, cn_outer = 0.0;
     int phase = kPhNeedPixel, count = 0, nev = 0, ls = 0, ocount = 0;
     int head = 0;  // history: slot of the oldest record of the ring
+    int dstep = 0;  // history: two-loop steps done of the pending direction
+    unsigned int freev = 0;  // free variables of the direction being formed
     bool warm_try = false;  // this pixel's solve started from the warm-start prediction
     long long pix = 0;
 #pragma unroll
@@ -788,7 +860,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         unsigned long long needy = __ballot(phase == kPhNeedPixel);
         const unsigned long long retry = __ballot(phase == kPhRetry);
         const bool serve = __popcll(needy | retry) >= kFetchBatch ||
-                           __ballot(phase == kPhStart || phase == kPhTrial || phase == kPhOuter) == 0;
+                           __ballot(phase == kPhStart || phase == kPhTrial || phase == kPhOuter || phase == kPhDir) == 0;
         if (!serve) needy = 0;
         long long item = -1;
         while (needy) {
@@ -852,7 +924,8 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         }
 
         // ---- a drained wave moves to its pending chunk, or ends ----
-        if (__ballot(phase == kPhStart || phase == kPhTrial || phase == kPhOuter || phase == kPhRetry) == 0) {
+        if (__ballot(phase == kPhStart || phase == kPhTrial || phase == kPhOuter || phase == kPhRetry ||
+                     phase == kPhDir) == 0) {
             if (!q.pending) {
                 q.img = -1;
                 return;
@@ -939,16 +1012,17 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                             slot = head;
                             head = (head + 1 == cap) ? 0 : head + 1;
                         }
-                        constexpr int R = 2 * N + 2;
+                        constexpr int NP = kHistPad(L);
+                        constexpr int R = kHistRecord(L);
                         double* const rec = (&args.hist)[zero] +
                                             ((long long)blockIdx.x * 64 + lane) * (long long)cap * R +
                                             (long long)slot * R;
 #pragma unroll
-                        for (int i = 0; i < N; ++i) {
-                            rec[i] = S[i];
-                            rec[N + i] = Y[i];
+                        for (int i = 0; i < NP; ++i) {
+                            rec[i] = (i < N) ? S[i] : 0.0;
+                            rec[NP + i] = (i < N) ? Y[i] : 0.0;
                         }
-                        rec[2 * N] = rh;
+                        rec[2 * NP] = rh;
                     } else {
                         count = 1;
                     }
@@ -977,10 +1051,9 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             }
         }
 
-        unsigned int freev = 0;
-        bool need_dir = false;  // history: the direction is formed by the wave below
         if (direction) {
             // free set and projected gradient; q is then turned into d in place
+            freev = 0;
             bool finite = __builtin_isfinite(f);
             double q[N];
             double pgmax = 0.0;
@@ -1003,7 +1076,8 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                 lc.pairs += (unsigned)count;
 #pragma unroll
                 for (int i = 0; i < N; ++i) dv[i] = q[i];
-                need_dir = true;
+                dstep = 0;
+                phase = kPhDir;
             } else {
                 ++lc.iters;
                 lc.pairs += (unsigned)count;
@@ -1047,14 +1121,15 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             }
         }
         if constexpr (HIST) {
-            const unsigned long long pend = __ballot(need_dir);
+            const unsigned long long pend = __ballot(phase == kPhDir);
+            unsigned long long done = 0;
             if (pend) {
-                constexpr int R = 2 * N + 2;
+                constexpr int R = kHistRecord(L);
                 const int cap = (&args.hist_cap)[zero];
                 const double* hblock = (&args.hist)[zero] + (long long)blockIdx.x * 64 * (long long)cap * R;
-                direction_service<N>(pend, smem + N * 64, hblock, cap, count, head, gamma);
+                done = direction_service<L>(pend, smem + N * 64, hblock, cap, count, head, gamma, dstep);
             }
-            if (need_dir) {
+            if ((done >> lane) & 1ull) {
                 double qd[N];
                 double gd = 0.0;
 #pragma unroll
@@ -1087,7 +1162,7 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
         if (inner_end) phase = kPhOuter;
         const unsigned long long outer_lanes = __ballot(phase == kPhOuter);
         const bool run_outer = __popcll(outer_lanes) >= kOuterBatch ||
-                               __ballot(phase == kPhStart || phase == kPhTrial) == 0;
+                               __ballot(phase == kPhStart || phase == kPhTrial || phase == kPhDir) == 0;
         if (run_outer && phase == kPhOuter) {
             lc.evals += (unsigned)nev;
             lc.outer += 1;
