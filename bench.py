@@ -3,16 +3,17 @@
 A step is one unmix + refine of one synthetic image resident in HBM (the
 default workload: 4096 x 4096, 7 layers -- Normal, Multiply, Screen,
 Overlay, ColorBurn, HardLight, SoftLight -- guided filter r = 32,
-eps = 1e-3).  The step is timed twice: plain (every pixel starts at the
-reference's initial point) and with the warm-start network, trained first on
-2^20 pixels of other images of the same layer set solved by the plain GPU
-path (Adam, batch 4096, 20k steps; the training time is reported, not
-counted in the step).  Prints one JSON line: `value` is the warm-started
-Mpixel/s, with the plain rate, the difference between the two outputs and
-their full-image reconstruction errors (device recomposite), the roofline of
-the dominant kernel (FP64 solve), the filter's HBM rate, the network's
-matrix-core rate, and the oracle timed on the host cores on a band of the
-same image (whose first pass and refinement both runs are compared with).
+eps = 1e-3).  `value` is the plain step with the default inner minimiser
+(NLopt's default L-BFGS history for LD_LBFGS: every (s, y) pair of an inner
+solve), whose outputs are bit-identical to the oracle's.  Reported beside
+it, each with its measured difference from that step's outputs: the
+one-pair fast mode (lbfgs_memory = 1) and the warm start from the network
+(trained first on other images of the same layer set solved by the plain
+GPU path; training time reported, not counted in the step).  Also: the
+roofline of the dominant kernel (the first-pass solve, bound by its (s, y)
+history traffic in HBM), the filter's HBM rate, the network's matrix-core
+rate, and the oracle timed on the host cores: c1 and c2 whole, and a band
+of this image, whose outputs are compared with the GPU's.
 """
 from __future__ import annotations
 
@@ -32,12 +33,27 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def solve_flops(evals: int, iterations: int, L: int) -> float:
+def solve_flops(evals: int, iterations: int, pairs: int, L: int) -> float:
     """Algorithmic FP64 operations of the solve kernels (add/sub/mul/div/sqrt
     = 1, counted from the device source): an objective + gradient evaluation
     with its trial point and Armijo product costs 277 L - 168; a search
-    direction (free set, two-loop over 4 pairs, update) 128 L."""
-    return float(evals) * (277 * L - 168) + float(iterations) * (128 * L)
+    direction outside its two-loop recursion (free set, scaling, negation,
+    descent test, step length) 92 L; each (s, y) pair the recursion goes
+    through 32 L + 17 (two dot products over N = 4L components with the
+    7 additions of their tree, two axpys, three scalar operations)."""
+    return float(evals) * (277 * L - 168) + float(iterations) * (92 * L) + float(pairs) * (32 * L + 17)
+
+
+def solve_bytes(n: int, pairs: int, iterations: int, L: int, refine: bool) -> float:
+    """Algorithmic HBM bytes of a solve launch: each pair the two-loop
+    recursion goes through is read twice (s, y and 1/s.y; 4L components of
+    each vector) with its multiplier written once and read once, each
+    accepted step writes one record (s, y, 1/s.y), and every pixel reads its
+    colour (24 B) and writes its packed layers (32 L B); the refinement also
+    reads the first pass's colours (24 L B) and the filtered alphas (8 L B)."""
+    N = 4 * L
+    per_pixel = 24 + 32 * L + (32 * L if refine else 0)
+    return 8.0 * (float(pairs) * (4 * N + 4) + float(iterations) * (2 * N + 1)) + float(n) * per_pixel
 
 
 def filter_bytes(n: int, channels: int) -> float:
@@ -55,17 +71,16 @@ def mlp_train_flops(batch: int) -> float:
     return 2.0 * batch * (2 * fwd + 32 * 128 + 2 * 128 * 128)
 
 
-def profiled_traffic(w, h, L, refine):
+def profiled_traffic(w, h, L, refine, memory):
     """HBM bytes per dispatch of the solve kernel, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or
     None when there is none for it."""
-    path = os.path.join(ROOT, "profiles", "bench_c%d_%d_hbm_traffic.json" % (3, w))
-    if (w, h, L) != (4096, 4096, 7) or not os.path.exists(path):
+    path = os.path.join(ROOT, "profiles", "solve_history_c3_4096_hbm_traffic.json")
+    if (w, h, L) != (4096, 4096, 7) or memory != 0 or not os.path.exists(path):
         return None, None
     with open(path) as f:
         prof = json.load(f)
-    key = "void solve_batch_kernel<%d, %s>(BatchArgs)" % (L, "true" if refine else "false")
-    k = prof["kernels"].get(key)
+    k = prof["kernels"].get("refinement" if refine else "first_pass")
     if not k:
         return None, None
     return k["bytes"], "profiles/" + os.path.basename(path)
@@ -94,24 +109,59 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, rows, threads, bounds=None):
-    """The oracle on rows 0..rows-1 of the image.  bounds: the exact filter's
-    bounds of the whole image (max|I|, max|alpha_l|), so that the band's
-    refinement uses the whole image's quantisation grids."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_pass_pair(wl, rows, threads, bounds=None):
+    """The oracle's unmix + refine of rows 0..rows-1 of a workload (all rows
+    when rows is None); returns (seconds, first, second).  bounds: the exact
+    filter's bounds of the whole image (max|I|, max|alpha_l|), so that a
+    band's refinement uses the whole image's quantisation grids."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ub_oracle as O
-    band = np.ascontiguousarray(wl.rgb[:rows])
+    band = np.ascontiguousarray(wl.rgb if rows is None else wl.rgb[:rows])
     layers = [O.Layer(l.mode, tuple(l.mean), tuple(np.ravel(l.precision)), l.comp) for l in wl.layers]
     t0 = time.perf_counter()
     first, _ = O.unmix(band, layers, opaque=True, threads=threads)
     second, _ = O.refine(band, layers, first, opaque=True, radius=wl.radius, eps=wl.eps, threads=threads,
                          bounds=bounds)
-    dt = time.perf_counter() - t0
-    n = band.shape[0] * band.shape[1]
+    return time.perf_counter() - t0, first, second
+
+
+def cpu_baseline(wl, rows, threads, bounds=None, extra_configs=True):
+    """The oracle on the host cores: a band of the benchmarked image (the
+    reported value, compared with the GPU's outputs), a single-thread rate on
+    a smaller band, and (extra_configs) the c1 and c2 workloads whole."""
+    from unblending_amd import synth
+    dt, first, second = oracle_pass_pair(wl, rows, threads, bounds)
+    n = rows * wl.width
+    srows = max(1, rows // 16)
+    dt1, _, _ = oracle_pass_pair(wl, srows, 1, bounds)
     rec = {"value": n / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
-           "target_concurrency": threads,
+           "target_concurrency": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+           "filter": "exact-sum guided filter (the device's definition; within 1e-9 of the FP64 brute-force one)",
+           "inner_minimiser": "L-BFGS with NLopt's default history (every pair of an inner solve)",
            "sample": "oracle (FP64 C restatement), unmix + refine of rows 0-%d of the same image "
-                     "(%d x %d px), %d threads, %.1f s" % (rows - 1, band.shape[1], rows, threads, dt)}
+                     "(%d x %d px), %d threads, %.1f s" % (rows - 1, wl.width, rows, threads, dt),
+           "single_thread": {"value": srows * wl.width / dt1 / 1e6, "unit": "Mpixel/s",
+                             "sample": "rows 0-%d (%d px), 1 thread, %.1f s" % (srows - 1, srows * wl.width, dt1)}}
+    if extra_configs:
+        whole = {}
+        for k in (1, 2):
+            wk = synth.config(k)
+            dk, _, _ = oracle_pass_pair(wk, None, threads)
+            whole["c%d" % k] = {"workload": wk.name, "value": wk.width * wk.height / dk / 1e6, "unit": "Mpixel/s",
+                                "seconds": round(dk, 2), "threads": threads}
+            log("oracle %s whole: %.1f s" % (wk.name, dk))
+        rec["whole_configs"] = whole
     return rec, first, second
 
 
@@ -168,11 +218,14 @@ def run_batch_bench(args):
     Ls = [len(wl.layers) for wl in wls]
     # FP64 work of the solves: per-image flops depend on L; use the pixel-weighted mean L
     Lbar = sum(len(wl.layers) * wl.width * wl.height for wl in wls) / n
-    f1 = float(s1.evals) * (277 * Lbar - 168) + float(s1.iterations) * 128 * Lbar
-    f2 = float(s2.evals) * (277 * Lbar - 168) + float(s2.iterations) * 128 * Lbar
-    dom, dflops, dms = (("refinement solve (FP64)", f2, s2.solve_ms) if s2.solve_ms >= s1.solve_ms
-                        else ("first-pass solve (FP64)", f1, s1.solve_ms))
-    achieved = dflops / (dms * 1e-3) / 1e12
+    if s2.solve_ms >= s1.solve_ms:
+        dom, ks, refine_dom = "refinement solve", s2, True
+    else:
+        dom, ks, refine_dom = "first-pass solve", s1, False
+    dms = ks.solve_ms
+    dflops = solve_flops(ks.evals, ks.iterations, ks.history_pairs, Lbar)
+    dbytes = solve_bytes(n, ks.history_pairs, ks.iterations, Lbar, refine_dom)
+    achieved = dbytes / (dms * 1e-3) / 1e9
     fbytes = sum(filter_bytes(wl.width * wl.height, len(wl.layers)) for wl in wls)
     rec = {
         "metric": "Mpixel/s unmix+refine, plain vs MLP warm-start at equal error; filter HBM GB/s",
@@ -192,9 +245,12 @@ def run_batch_bench(args):
             "layers_histogram": {str(L): Ls.count(L) for L in sorted(set(Ls))}, "filter_eps": 1e-4,
             "filter_radius": "min(w, h) / 64", "warm_start": "none", "graph": "both passes, one HIP graph replay a step",
             "chunk_size": chunk, "chunk_tune_s": round(tune_s, 2)},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "launch_ms": round(dms, 3), "flops_per_launch": dflops},
+        "roofline": {"bound": "hbm", "kernel": dom + " (all launches of the pass; pixel-weighted mean layer count)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": dbytes,
+                     "launch_ms": round(dms, 3),
+                     "fp64": {"achieved": round(dflops / (dms * 1e-3) / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "flops": dflops}},
         "filter": {"achieved": round(fbytes / (s2.filter_ms * 1e-3) / 1e9, 1) if s2.filter_ms > 0 else 0.0,
                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "ms": round(s2.filter_ms, 3), "launches": s2.filter_launches},
         "solve": {"unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
@@ -208,7 +264,7 @@ def run_batch_bench(args):
         bounds = np.array([float(rgbs[0].abs().max())] + [float(first[0].out[l, ..., 3].abs().max())
                                                            for l in range(L0)])
         rows = min(args.cpu_rows, wl.height)
-        cpu, ofirst, osecond = cpu_baseline(wl, rows, threads, bounds)
+        cpu, ofirst, osecond = cpu_baseline(wl, rows, threads, bounds, extra_configs=False)
         keep = max(0, rows - 2 * wl.radius - 2) if rows < wl.height else rows
         g1 = first[0].out[:, :rows].cpu().numpy()
         g2 = second[0].out[:, :keep].cpu().numpy()
@@ -226,12 +282,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--size", type=str, default="")
-    ap.add_argument("--cpu-rows", type=int, default=640)
+    ap.add_argument("--cpu-rows", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solver", type=int, default=0)
     ap.add_argument("--lbfgs-memory", type=int, default=0,
                     help="(s, y) pairs of the inner minimiser: 0 = NLopt's default (every pair), 1 = fast mode")
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-fast", action="store_true", help="skip the one-pair fast-mode comparison")
+    ap.add_argument("--no-cpu-configs", action="store_true", help="skip timing the oracle on c1 and c2 whole")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-size", type=int, default=512)
     ap.add_argument("--train-images", type=int, default=4)
@@ -289,38 +347,39 @@ def main():
         chunk, _ = autotune.tune_chunk(rgb, infos, log=log)
         shape = {"chunk_size": chunk, "filter_block": block, "filter_rows": rows}
     tune_s = time.perf_counter() - t0
-    params = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape, **mem)
 
-    def step(p=params, s1=None, s2=None, o=out):
-        device.unmix(rgb, infos, True, out=first, alpha_planes=alpha, ws=ws1, params=p, stats=s1)
-        device.refine(rgb, first, infos, True, False, alpha_planes=alpha, out=o, ws=ws2, params=p, stats=s2)
+    def make_params(**kw):
+        kw.setdefault("lbfgs_memory", args.lbfgs_memory)
+        return U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape, **kw)
 
-    def timed(p, o, label):
-        for i in range(args.warmup):
-            step(p, o=o)
+    params = make_params()
+
+    def step(p, s1=None, s2=None, f=first, o=out):
+        device.unmix(rgb, infos, True, out=f, alpha_planes=alpha, ws=ws1, params=p, stats=s1)
+        device.refine(rgb, f, infos, True, False, alpha_planes=alpha, out=o, ws=ws2, params=p, stats=s2)
+
+    def timed(p, label, steps, warmup, f=first, o=out):
+        for i in range(warmup):
+            step(p, f=f, o=o)
             torch.cuda.synchronize()
             log("%s warmup step %d done" % (label, i))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for i in range(args.steps):
-            step(p, o=o)
+        for i in range(steps):
+            step(p, f=f, o=o)
         torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        log("%s: timed %d steps, %.2f ms/step" % (label, args.steps, ms))
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        log("%s: timed %d steps, %.2f ms/step" % (label, steps, ms))
         return ms
 
-    def instrumented(p, o):
+    def instrumented(p, f=first, o=out):
         """One step with HIP events around each kernel on the launch stream."""
         p.timing = 1
         s1, s2 = U.HipStats(), U.HipStats()
-        step(p, s1, s2, o=o)
+        step(p, s1, s2, f=f, o=o)
         torch.cuda.synchronize()
         p.timing = 0
         return s1, s2
-
-    ms_plain = timed(params, out, "plain")
-    s1, s2 = instrumented(U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, **shape, **mem), out)
-    first_plain = first.clone()
 
     def recon(packed):
         """RMS of (recomposite RGB - image) over the whole image, alpha error
@@ -329,8 +388,42 @@ def main():
         err = rgba[..., :3] - rgb
         return {"rgb_rms": float(err.square().mean().sqrt()), "alpha_max_err": float((rgba[..., 3] - 1.0).abs().max())}, rgba
 
+    def differs(a, b):
+        d = (a - b).abs()
+        return {"max": float(d.max()), "mean": float(d.mean()), "frac_gt_1e-3": float((d > 1e-3).double().mean())}
+
+    # ---- the reported step: plain, default minimiser ----
+    ms_plain = timed(params, "plain", args.steps, args.warmup)
+    s1, s2 = instrumented(make_params())
+    first_plain, out_plain = first.clone(), out.clone()
     recon_plain_first, rc_first = recon(first_plain)
-    recon_plain_refined, rc_refined = recon(out)
+    recon_plain_refined, rc_refined = recon(out_plain)
+    side_steps, side_warm = max(1, min(args.steps, 5)), min(args.warmup, 1)
+
+    # ---- the one-pair fast mode, against the reported step's outputs ----
+    fast = None
+    if args.lbfgs_memory != 1 and not args.no_fast:
+        fparams = make_params(lbfgs_memory=1)
+        wsf1 = device.workspace(w, h, L, False, device=dev, params=fparams)
+        wsf2 = device.workspace(w, h, L, True, device=dev, params=fparams)
+        ws1_keep, ws2_keep = ws1, ws2
+        ws1, ws2 = wsf1, wsf2
+        f_first, f_out = torch.empty_like(first), torch.empty_like(out)
+        ms_fast = timed(fparams, "fast (one pair)", side_steps, side_warm, f=f_first, o=f_out)
+        fs1, fs2 = instrumented(make_params(lbfgs_memory=1), f=f_first, o=f_out)
+        ws1, ws2 = ws1_keep, ws2_keep
+        del wsf1, wsf2
+        fast = {"lbfgs_memory": 1, "value": round(n / (ms_fast * 1e-3) / 1e6, 4), "ms_per_step": round(ms_fast, 3),
+                "speedup_vs_value": round(ms_plain / ms_fast, 4),
+                "unmix_ms": round(fs1.solve_ms, 3), "refine_ms": round(fs2.solve_ms, 3),
+                "unmix_evals_per_px": fs1.evals / n, "refine_evals_per_px": fs2.evals / n,
+                "vs_default_first_pass": differs(f_first, first_plain),
+                "vs_default_refined": differs(f_out, out_plain),
+                "equal_error": False,
+                "note": "a different inner minimiser (one pair instead of NLopt's default history): a different "
+                        "method whose outputs differ from the oracle's, reported for its speed only"}
+        fast["equal_error"] = fast["vs_default_first_pass"]["max"] <= 1e-3 and fast["vs_default_refined"]["max"] <= 1e-3
+        del f_first, f_out
 
     warm = None
     if not args.no_warm:
@@ -341,11 +434,11 @@ def main():
         log("generated %d training images %dx%d in %.1f s" % (len(tw), ts, ts, time.perf_counter() - t0))
         images = torch.from_numpy(np.stack([t.rgb for t in tw])).to(dev).contiguous()
         targets = torch.empty((len(tw), L, ts, ts, 4), dtype=torch.float64, device=dev)
-        wst = device.workspace(ts, ts, L, False, device=dev)
+        wst = device.workspace(ts, ts, L, False, device=dev, params=params)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for j in range(len(tw)):
-            device.unmix(images[j], infos, True, out=targets[j], ws=wst)
+            device.unmix(images[j], infos, True, out=targets[j], ws=wst, params=params)
         torch.cuda.synchronize()
         label_s = time.perf_counter() - t0
         model = mlp.WarmStartModel(L, device=dev, seed=0)
@@ -357,46 +450,58 @@ def main():
         log("trained %d steps in %.1f s, loss %.3e -> %.3e"
             % (args.train_steps, train_s, float(np.mean(losses[:10])), float(np.mean(losses[-100:]))))
         del images, targets, wst
-        wparams = U.hip_params(radius=wl.radius, eps=wl.eps, solver=args.solver, warm_start=model, **shape)
-        ws1 = device.workspace(w, h, L, False, device=dev, warm=True)
-        out_warm = torch.empty_like(out)
-        ms_warm = timed(wparams, out_warm, "warm")
-        w1, w2 = instrumented(wparams, out_warm)
-        recon_warm_first, rw_first = recon(first)
-        recon_warm_refined, rw_refined = recon(out_warm)
-        warm_recon = {"first": recon_warm_first, "refined": recon_warm_refined,
-                      "vs_plain_first_rms": float((rw_first - rc_first).square().mean().sqrt()),
-                      "vs_plain_refined_rms": float((rw_refined - rc_refined).square().mean().sqrt())}
-        del rw_first, rw_refined
-        d1 = (first - first_plain).abs()
-        d2 = (out_warm - out).abs()
-        per_train = train_s / max(args.train_steps, 1)
+        wparams = make_params(warm_start=model)
+        w_first, w_out = torch.empty_like(first), torch.empty_like(out)
+        ms_warm = timed(wparams, "warm", side_steps, side_warm, f=w_first, o=w_out)
+        w1, w2 = instrumented(make_params(warm_start=model), f=w_first, o=w_out)
+        recon_warm_first, rw_first = recon(w_first)
+        recon_warm_refined, rw_refined = recon(w_out)
         band = min(h, 512)
         qp = first_pass_quality(first_plain[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
-        qw = first_pass_quality(first[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
-        warm = {"ms_per_step": ms_warm, "s1": w1, "s2": w2, "train_s": train_s, "label_s": label_s,
-                "quality": {"rows": band, "plain": qp, "warm": qw}, "recomposite": warm_recon,
-                "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-100:])),
-                "train_tflops": mlp_train_flops(args.train_batch) / per_train / 1e12 if per_train > 0 else 0.0,
-                "first_max": float(d1.max()), "first_mean": float(d1.mean()),
-                "first_frac_gt_1e-3": float((d1 > 1e-3).double().mean()),
-                "refined_max": float(d2.max()), "refined_mean": float(d2.mean()),
-                "refined_frac_gt_1e-3": float((d2 > 1e-3).double().mean())}
+        qw = first_pass_quality(w_first[:, :band].cpu().numpy(), wl.rgb[:band], wl.layers)
+        per_train = train_s / max(args.train_steps, 1)
+        dfirst, drefined = differs(w_first, first_plain), differs(w_out, out_plain)
+        warm = {
+            "value": round(n / (ms_warm * 1e-3) / 1e6, 4), "ms_per_step": round(ms_warm, 3),
+            "speedup_vs_value": round(ms_plain / ms_warm, 4),
+            "unmix_ms": round(w1.solve_ms, 3), "predict_ms": round(w1.predict_ms, 3),
+            "refine_ms": round(w2.solve_ms, 3),
+            "unmix_evals_per_px": w1.evals / n, "unmix_iters_per_px": w1.iterations / n,
+            "evals_saved_frac": round(1.0 - w1.evals / s1.evals, 4),
+            "iterations_saved_frac": round(1.0 - w1.iterations / s1.iterations, 4),
+            "fallback_pixels": int(w1.fallback_pixels),
+            "refine_evals_per_px": w2.evals / n,
+            "vs_default_first_pass": dfirst, "vs_default_refined": drefined,
+            "equal_error": dfirst["max"] <= 1e-3 and drefined["max"] <= 1e-3,
+            "first_pass_quality": {"rows": band, "plain": qp, "warm": qw},
+            "recomposite": {"first": recon_warm_first, "refined": recon_warm_refined,
+                            "vs_plain_first_rms": float((rw_first - rc_first).square().mean().sqrt()),
+                            "vs_plain_refined_rms": float((rw_refined - rc_refined).square().mean().sqrt())},
+            "training": {"images": args.train_images, "size": args.train_size, "steps": args.train_steps,
+                         "batch": args.train_batch, "lr": args.train_lr, "label_s": round(label_s, 3),
+                         "train_s": round(train_s, 3), "loss_first": float(np.mean(losses[:10])),
+                         "loss_last": float(np.mean(losses[-100:])),
+                         "mfma_tflops": round(mlp_train_flops(args.train_batch) / per_train / 1e12, 3),
+                         "mfma_frac_bf16": round(mlp_train_flops(args.train_batch) / per_train / 1e12
+                                                 / BF16_PEAK_TFLOPS, 5)},
+            "note": "the warm-started solve converges to other accepted points of the per-pixel problem; "
+                    "it is reported beside the value, not as the value, unless equal_error holds"}
+        del rw_first, rw_refined
 
-    flops1 = solve_flops(s1.evals, s1.iterations, L)
-    flops2 = solve_flops(s2.evals, s2.iterations, L)
-    fbytes = filter_bytes(n, s2.filter_channels)
-    fgbs = fbytes / (s2.filter_ms * 1e-3) / 1e9 if s2.filter_ms > 0 else 0.0
-    # the dominant kernel of the reported step: the longer of the two solves
-    k1, k2 = (warm["s1"], warm["s2"]) if warm else (s1, s2)
-    if k2.solve_ms >= k1.solve_ms:
-        dom, dflops, dms = "refinement solve (FP64)", solve_flops(k2.evals, k2.iterations, L), k2.solve_ms
+    # ---- roofline of the dominant kernel of the reported step ----
+    fb = filter_bytes(n, s2.filter_channels)
+    filter_ms = s2.filter_ms  # bounds pass + pass A + pass B
+    fgbs = fb / (filter_ms * 1e-3) / 1e9 if filter_ms > 0 else 0.0
+    if s1.solve_ms >= s2.solve_ms:
+        dom, ks, refine_dom = "first-pass solve", s1, False
     else:
-        dom, dflops, dms = "first-pass solve (FP64)", solve_flops(k1.evals, k1.iterations, L), k1.solve_ms
-    achieved = dflops / (dms * 1e-3) / 1e12
-    traffic, traffic_source = profiled_traffic(w, h, L, "refinement" in dom)
-    ms = warm["ms_per_step"] if warm else ms_plain
-    value = n / (ms * 1e-3) / 1e6
+        dom, ks, refine_dom = "refinement solve", s2, True
+    dflops = solve_flops(ks.evals, ks.iterations, ks.history_pairs, L)
+    dbytes = solve_bytes(n, ks.history_pairs, ks.iterations, L, refine_dom)
+    gbs = dbytes / (ks.solve_ms * 1e-3) / 1e9
+    tflops = dflops / (ks.solve_ms * 1e-3) / 1e12
+    traffic, traffic_source = profiled_traffic(w, h, L, refine_dom, args.lbfgs_memory)
+    value = n / (ms_plain * 1e-3) / 1e6
 
     rec = {
         "metric": "Mpixel/s unmix+refine, plain vs MLP warm-start at equal error; filter HBM GB/s",
@@ -405,7 +510,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(ms_plain, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -413,73 +518,55 @@ def main():
         "data": "synthetic (seeded generator, unblending_amd/synth.py)",
         "config": {"workload": wl.name, "width": w, "height": h, "layers": L,
                    "modes": [synth.MODE_NAMES[l.mode] for l in wl.layers], "filter_radius": wl.radius,
-                   "filter_eps": wl.eps, "warm_start": "mlp" if warm else "none",
+                   "filter_eps": wl.eps, "warm_start": "none (reported under warm_start)",
+                   "lbfgs_memory": args.lbfgs_memory or "NLopt default (every pair of an inner solve)",
                    "solver": "persistent" if args.solver == 0 else "simple",
                    "launch_shape": shape, "autotune_s": round(tune_s, 2)},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": traffic, "traffic_source": traffic_source, "launch_ms": round(dms, 3),
-                     "flops_per_launch": dflops},
+        "roofline": {"bound": "hbm", "kernel": dom + " (FP64, (s, y) history streamed from HBM)",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_source,
+                     "bytes_per_launch": dbytes, "launch_ms": round(ks.solve_ms, 3),
+                     "fp64": {"achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flops_per_launch": dflops}},
         "filter": {"achieved": round(fgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": round(fgbs / HBM_PEAK_GBS, 4), "ms": round(s2.filter_ms, 3),
+                   "frac": round(fgbs / HBM_PEAK_GBS, 4), "ms": round(filter_ms, 3),
                    "coeff_ms": round(s2.filter_coeff_ms, 3), "out_ms": round(s2.filter_out_ms, 3),
-                   "bytes": fbytes, "channels": s2.filter_channels},
-        "plain": {"value": round(n / (ms_plain * 1e-3) / 1e6, 4), "ms_per_step": round(ms_plain, 3),
-                  "unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
+                   "bytes": fb, "channels": s2.filter_channels,
+                   "timed": "bounds pass + pass A + pass B"},
+        "plain": {"unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
                   "unmix_evals_per_px": s1.evals / n, "unmix_iters_per_px": s1.iterations / n,
                   "unmix_outer_per_px": s1.outer / n, "refine_evals_per_px": s2.evals / n,
                   "refine_iters_per_px": s2.iterations / n, "refine_outer_per_px": s2.outer / n,
                   "unmix_pairs_per_px": s1.history_pairs / n, "refine_pairs_per_px": s2.history_pairs / n,
                   "unmix_lane_use": s1.evals / s1.lane_slots if s1.lane_slots else None,
                   "refine_lane_use": s2.evals / s2.lane_slots if s2.lane_slots else None,
-                  "unmix_tflops": round(flops1 / (s1.solve_ms * 1e-3) / 1e12, 3),
-                  "refine_tflops": round(flops2 / (s2.solve_ms * 1e-3) / 1e12, 3),
+                  "unmix_gbs": round(solve_bytes(n, s1.history_pairs, s1.iterations, L, False)
+                                     / (s1.solve_ms * 1e-3) / 1e9, 1),
+                  "refine_gbs": round(solve_bytes(n, s2.history_pairs, s2.iterations, L, True)
+                                      / (s2.solve_ms * 1e-3) / 1e9, 1),
+                  "unmix_tflops": round(solve_flops(s1.evals, s1.iterations, s1.history_pairs, L)
+                                        / (s1.solve_ms * 1e-3) / 1e12, 3),
+                  "refine_tflops": round(solve_flops(s2.evals, s2.iterations, s2.history_pairs, L)
+                                         / (s2.solve_ms * 1e-3) / 1e12, 3),
                   "recomposite": {"first": recon_plain_first, "refined": recon_plain_refined}},
     }
+    if fast:
+        rec["fast_mode"] = fast
     if warm:
-        w1, w2 = warm["s1"], warm["s2"]
-        rec["warm_start"] = {
-            "speedup": round(ms_plain / warm["ms_per_step"], 4),
-            "unmix_ms": round(w1.solve_ms, 3), "predict_ms": round(w1.predict_ms, 3),
-            "refine_ms": round(w2.solve_ms, 3),
-            "unmix_evals_per_px": w1.evals / n, "unmix_iters_per_px": w1.iterations / n,
-            "evals_saved_frac": round(1.0 - w1.evals / s1.evals, 4),
-            "iterations_saved_frac": round(1.0 - w1.iterations / s1.iterations, 4),
-            "fallback_pixels": int(w1.fallback_pixels),
-            "refine_evals_per_px": w2.evals / n,
-            "vs_plain_first_pass": {"max": warm["first_max"], "mean": warm["first_mean"],
-                                    "frac_gt_1e-3": warm["first_frac_gt_1e-3"]},
-            "vs_plain_refined": {"max": warm["refined_max"], "mean": warm["refined_mean"],
-                                 "frac_gt_1e-3": warm["refined_frac_gt_1e-3"]},
-            "first_pass_quality": warm["quality"],
-            "recomposite": warm["recomposite"],
-            "training": {"images": args.train_images, "size": args.train_size, "steps": args.train_steps,
-                         "batch": args.train_batch, "lr": args.train_lr, "label_s": round(warm["label_s"], 3),
-                         "train_s": round(warm["train_s"], 3), "loss_first": warm["loss_first"],
-                         "loss_last": warm["loss_last"], "mfma_tflops": round(warm["train_tflops"], 3),
-                         "mfma_frac_bf16": round(warm["train_tflops"] / BF16_PEAK_TFLOPS, 5)},
-        }
+        rec["warm_start"] = warm
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         bounds = np.array([float(rgb.abs().max())] + [float(first_plain[l, ..., 3].abs().max()) for l in range(L)])
-        cpu, ofirst, osecond = cpu_baseline(wl, args.cpu_rows, threads, bounds)
+        cpu, ofirst, osecond = cpu_baseline(wl, args.cpu_rows, threads, bounds, extra_configs=not args.no_cpu_configs)
         rows = args.cpu_rows
         keep = max(0, rows - 2 * wl.radius - 2)  # refinement rows unaffected by the band's lower edge
         g1 = first_plain[:, :rows].cpu().numpy()
-        g2 = out[:, :keep].cpu().numpy()
+        g2 = out_plain[:, :keep].cpu().numpy()
         cpu["plain_vs_oracle"] = {"first_max": float(np.abs(g1 - ofirst).max()),
                                   "refined_max": float(np.abs(g2 - osecond[:, :keep]).max()) if keep else None,
                                   "refined_rows": keep,
                                   "bit_identical": bool(np.array_equal(g1, ofirst) and
                                                         np.array_equal(g2, osecond[:, :keep]))}
-        if warm:
-            gw1 = first[:, :rows].cpu().numpy()
-            gw2 = out_warm[:, :keep].cpu().numpy()
-            cpu["warm_vs_oracle"] = {"first_max": float(np.abs(gw1 - ofirst).max()),
-                                     "first_frac_gt_1e-3": float(np.mean(np.abs(gw1 - ofirst) > 1e-3)),
-                                     "refined_max": float(np.abs(gw2 - osecond[:, :keep]).max()) if keep else None,
-                                     "refined_frac_gt_1e-3":
-                                         float(np.mean(np.abs(gw2 - osecond[:, :keep]) > 1e-3)) if keep else None}
         rec["cpu_baseline"] = cpu
     print(json.dumps(rec), flush=True)
 
