@@ -131,6 +131,21 @@ UNBLENDING_API int32_t unblending_hip_guided_filter_device(
 UNBLENDING_API size_t unblending_hip_filter_workspace_size(int32_t width, int32_t height,
                                                           int32_t channels);
 
+/* The guided filter without a workspace argument and without any device
+ * allocation: no counterpart in the reference.  For callers that cannot
+ * allocate -- code under HIP graph capture, serving loops that forbid
+ * hipMalloc.  Results equal unblending_hip_guided_filter_device's for the same
+ * arguments.  The filter's scratch (8 (4 channels width height + 32) bytes)
+ * lives in a device array of 256 MiB that the library holds on each GPU it
+ * runs on until the process ends; larger calls return
+ * UNBLENDING_INVALID_ARGUMENT (unblending_hip_last_error says why).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race.
+ * stats (may be NULL) receives the call's shape, no device times. */
+UNBLENDING_API int32_t unblending_hip_guided_filter_device_noalloc(
+    const double* rgb, int32_t width, int32_t height, const double* const* p, double* const* q,
+    int32_t channels, int32_t radius, double epsilon, int32_t block, int32_t rows, void* stream,
+    unblending_hip_stats* stats);
+
 /* ---- warm-start model (no counterpart in the reference) ----
  * A 27 -> 128 -> 128 -> 128 -> outputs network (GELU, sigmoid outputs) whose
  * input is a pixel's 3x3 RGB neighbourhood and whose outputs are the packed

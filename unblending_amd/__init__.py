@@ -217,6 +217,12 @@ _lib.unblending_hip_guided_filter_device.argtypes = [_vp, ctypes.c_int32, ctypes
                                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int32,
                                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                                      _vp, ctypes.c_size_t, _vp, ctypes.POINTER(HipStats)]
+_lib.unblending_hip_guided_filter_device_noalloc.restype = ctypes.c_int32
+_lib.unblending_hip_guided_filter_device_noalloc.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32,
+                                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                                             ctypes.c_int32, ctypes.c_int32, _vp,
+                                                             ctypes.POINTER(HipStats)]
 _lib.uhip_solve_planes.restype = ctypes.c_int32
 _lib.uhip_solve_planes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CLayer), ctypes.c_size_t,
                                    _dp, _dp, ctypes.c_int32, ctypes.c_int32, _dp]

@@ -480,6 +480,43 @@ UNBLENDING_API int32_t unblending_hip_guided_filter_device(const double* rgb, in
     }
 }
 
+UNBLENDING_API int32_t unblending_hip_guided_filter_device_noalloc(const double* rgb, int32_t width, int32_t height,
+                                                                   const double* const* p, double* const* q,
+                                                                   int32_t channels, int32_t radius, double epsilon,
+                                                                   int32_t block, int32_t rows, void* stream,
+                                                                   unblending_hip_stats* stats)
+{
+    g_last_error.clear();
+    try {
+        if (!rgb || !p || !q || channels <= 0 || channels > kMaxChannels || width <= 0 || height <= 0 ||
+            width > std::numeric_limits<int>::max() / height || radius < 0 || !(epsilon > 0.0) ||
+            !filter_block_ok(block, radius) || rows < 0)
+            return UNBLENDING_INVALID_ARGUMENT;
+        const size_t need = filter_workspace_bytes(size_t(width) * size_t(height), channels);
+        if (need > kNoAllocScratchBytes) {
+            g_last_error = "the filter's scratch (" + std::to_string(need) + " bytes for " +
+                           std::to_string(channels) + " channels of " + std::to_string(width) + " x " +
+                           std::to_string(height) + " pixels) exceeds the allocation-free entry's " +
+                           std::to_string(kNoAllocScratchBytes) + " bytes";
+            return UNBLENDING_INVALID_ARGUMENT;
+        }
+        void* ws = nullptr;
+        const hipError_t e = noalloc_scratch(&ws);
+        if (e != hipSuccess) throw ExecutionError(std::string("scratch address: ") + hipGetErrorString(e));
+        Channel in[kMaxChannels];
+        for (int c = 0; c < channels; ++c) in[c] = Channel{p[c], 1};
+        run_filter(rgb, width, height, in, q, channels, radius, epsilon, static_cast<double*>(ws),
+                   static_cast<hipStream_t>(stream), stats, false, block, rows);
+        return UNBLENDING_OK;
+    } catch (const ExecutionError& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    }
+}
+
 // Arguments of the recomposite entries: dimensions, 1 to 16 layers, comp-op
 // factors and blend modes of the reference's enums; the sizes in doubles.
 static int32_t composite_args_ok(const double* layers, int32_t width, int32_t height, size_t count,

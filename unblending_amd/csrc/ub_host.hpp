@@ -84,6 +84,11 @@ void host_composite(const double* layers, int w, int h, size_t count, const int3
                     double* out);
 
 hipError_t launch_zero(unsigned long long* p, int count, hipStream_t stream);
+
+// The allocation-free filter entry's device scratch (ub_noalloc.hip): its size,
+// and its device address on the current device (looked up once per device).
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+hipError_t noalloc_scratch(void** ptr);
 hipError_t launch_store_desc(ImageDesc* dst, const ImageDesc& d, hipStream_t stream);
 
 }  // namespace ubhip
