@@ -257,6 +257,7 @@ def test_warm_unmix_matches_oracle(oracle, c1_model):
     print("warm unmix max|diff| %.3e bit-identical pixels %.4f fallback %d (oracle %d)"
           % (d.max(), exact, stats.fallback_pixels, capped), flush=True)
     assert d.max() <= ALPHA_COLOR_TOL
+    assert d.max() == 0.0   # same arithmetic, same order: bit for bit
     assert stats.fallback_pixels == capped
     assert stats.pixels == w * h
 
@@ -282,3 +283,61 @@ def test_warm_start_saves_evaluations(c1_model):
     print("recomposite rms vs image: plain %.3e warm %.3e" % (ep, ew), flush=True)
     assert ew <= 1.5 * ep + 1e-4
     assert ratio < 0.9
+
+
+@pytest.fixture(scope="module")
+def c4_model():
+    """The warm-start configuration c4: c2's 5-layer set (Normal, Multiply,
+    Screen, ColorDodge, Overlay) and generator, a head of 4 L = 20 outputs in
+    the network's 32-wide output tile, trained on images of that generator."""
+    from unblending_amd import mlp
+    wls = _training_set(4, (64, 48), [41, 42, 43])
+    assert len(wls[0].layers) == 5
+    images, targets = _targets(wls)
+    model = mlp.WarmStartModel(5, seed=4)
+    model.fit(images, targets, steps=800, batch=2048, lr=3e-3, chunk=200)
+    return model
+
+
+def test_c4_warm_unmix_matches_oracle(oracle, c4_model):
+    """c4 at a reduced size: the warm-started first pass equals the oracle's
+    warm solve from the same predictions (pixels whose warm attempt hits its
+    outer cap solved again from the default start), with the same fallback
+    count, and the warm start saves evaluations over the plain solve."""
+    import unblending_amd as U
+    from unblending_amd import device, synth
+    wl = synth.config(4, (36, 28), image_seed=0x5EED0004)
+    L = len(wl.layers)
+    print("case warm unmix c4 36x28 L=%d" % L, flush=True)
+    rgb = _dev(wl.rgb)
+    infos = layer_infos(wl.layers)
+    pred = c4_model.predict(rgb).cpu().numpy()
+    assert pred.shape == (L, 28, 36, 4)
+    stats, plain_stats = U.HipStats(), U.HipStats()
+    out = device.unmix(rgb, infos, params=U.hip_params(warm_start=c4_model), stats=stats).cpu().numpy()
+    device.unmix(rgb, infos, stats=plain_stats)
+    ol = oracle_layers(wl.layers)
+    h, w, _ = wl.rgb.shape
+    ref = np.empty_like(out)
+    capped = 0
+    warm_settings = oracle.default_settings()
+    warm_settings.max_count = WARM_OUTER_CAP
+    for y in range(h):
+        for x in range(w):
+            x0 = np.concatenate([pred[:, y, x, 3], pred[:, y, x, :3].reshape(-1)])
+            sol, _, cap = oracle.solve_pixel_warm(wl.rgb[y, x], ol, x0, settings=warm_settings)
+            if cap:
+                capped += 1
+                sol, _ = oracle.solve_pixel(wl.rgb[y, x], ol)
+            ref[:, y, x, :3] = sol[L:].reshape(L, 3)
+            ref[:, y, x, 3] = sol[:L]
+    d = np.abs(out - ref)
+    print("c4 warm unmix max|diff| %.3e fallback %d (oracle %d) evals warm %d plain %d"
+          % (d.max(), stats.fallback_pixels, capped, stats.evals, plain_stats.evals), flush=True)
+    exact = float(np.mean(np.all(d == 0, axis=(0, 3))))
+    print("bit-identical pixels %.4f" % exact, flush=True)
+    assert d.max() <= ALPHA_COLOR_TOL
+    assert d.max() == 0.0   # same arithmetic, same order: bit for bit
+    assert stats.fallback_pixels == capped
+    assert stats.pixels == w * h
+    assert stats.evals < plain_stats.evals

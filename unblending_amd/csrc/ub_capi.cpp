@@ -579,9 +579,13 @@ UNBLENDING_API int32_t uhip_solve_planes(int32_t width, int32_t height, const un
 {
     g_last_error.clear();
     try {
-        if (width <= 0 || height <= 0 || !layers || !layer_count || !rgb_planes || !out_planes)
+        // the checks of unblending_c.cpp:56-63 before anything is allocated
+        constexpr auto max_int = std::numeric_limits<int>::max();
+        if (width <= 0 || height <= 0 || width > max_int / height || !layers || !layer_count ||
+            layer_count > size_t(max_int) / 4 || !rgb_planes || !out_planes)
             return UNBLENDING_INVALID_ARGUMENT;
         const size_t n = size_t(width) * size_t(height), L = layer_count;
+        if (n > std::vector<double>().max_size() / 4 / L) return UNBLENDING_INVALID_ARGUMENT;
         // gpu_dispatch.cpp:136-158 planes -> packed, and back (:182-191).
         std::vector<double> rgb(3 * n), first, out(4 * L * n);
         for (size_t i = 0; i < n; ++i)
@@ -612,7 +616,16 @@ UNBLENDING_API int32_t uhip_solve_planes(int32_t width, int32_t height, const un
             }
         return UNBLENDING_OK;
     } catch (const std::bad_alloc&) {
+        g_last_error = "out of memory";
         return UNBLENDING_OUT_OF_MEMORY;
+    } catch (const std::length_error& e) {
+        g_last_error = e.what();
+        return UNBLENDING_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_SOLVER_ERROR;
+    } catch (...) {
+        return UNBLENDING_SOLVER_ERROR;
     }
 }
 
