@@ -56,10 +56,17 @@ typedef struct unblending_hip_params {
                                  maxeval 1000 keeps every pair of an inner solve; the history lives in
                                  the workspace.  1: one pair in registers, the fast mode (its results
                                  differ from the default's).  n > 1: n pairs. */
+    int32_t gray_layers;      /* bit l: layer l's colour constrained to gray (sqrt(3) c - |c| 1 = 0), the
+                                 reference's SECOND_LAYER_GRAY build (unblending.cpp:21,168-172), which
+                                 is bit 1; the device solves 0 and bit 1 (2 to 8 layers, lbfgs_memory
+                                 != 1).  0: none. */
+    int32_t sparsity;         /* 1: the sparsity term 10 (sum alpha / sum alpha^2 - 1) in the first
+                                 pass's energy, the reference's SPARCITY build (unblending.cpp:17,
+                                 234-240).  0: off. */
 } unblending_hip_params;
 
 #define UNBLENDING_HIP_PARAMS_INIT \
-    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL, 0, 0, 0, 0, 0 }
+    { (uint32_t)sizeof(unblending_hip_params), -1, 0.0, 0, 0, 0, 0, NULL, 0, 0, 0, 0, 0, 0, 0 }
 
 typedef struct unblending_hip_stats {
     uint64_t evals;        /* objective+gradient evaluations, summed over pixels */

@@ -50,7 +50,8 @@ extern "C" {
 
 #define ORC_MAX_LAYERS 16
 #define ORC_MAX_VARS (4 * ORC_MAX_LAYERS)
-#define ORC_MAX_CONSTRAINTS (3 + ORC_MAX_LAYERS)
+#define ORC_MAX_CONSTRAINTS (3 + 4 * ORC_MAX_LAYERS) /* colour, alpha and gray rows */
+#define ORC_SQRT3 1.7320508075688772                   /* std::sqrt(3), correctly rounded */
 
 /* Same field meaning as unblending_layer (unblending/include/unblending.h:67-72). */
 typedef struct orc_layer {
@@ -77,7 +78,20 @@ typedef struct orc_settings {
     int32_t alpha_fixed;   /* refinement: 0 = alphas held by equality constraints
                               (reference CPU), 1 = alphas fixed by bounds
                               (reference MATTE_ALPHA_FORCE_FIXED, unblending.cpp:177-183) */
+    int32_t gray_mask;     /* bit l: layer l's colour constrained to gray (the reference's
+                              SECOND_LAYER_GRAY build: bit 1, unblending.cpp:168-172); 0 */
+    int32_t sparsity;      /* 1: the sparsity term in the first pass's energy (the reference's
+                              SPARCITY build, unblending.cpp:236-240); 0 */
+    double sigma;          /* sparsity weight (unblending.cpp:234: 10) */
 } orc_settings;
+
+/* The problem variants of the reference's build flags, as the pixel math
+ * takes them (sparsity is turned off by the solver in refinement). */
+typedef struct orc_variant {
+    int32_t gray_mask;
+    int32_t sparsity;
+    double sigma;
+} orc_variant;
 
 /* Per-pixel work counters (summed by the pass drivers). */
 typedef struct orc_counters {
@@ -133,6 +147,21 @@ int orc_constraint_jacobian(const double* x, int L, const orc_layer* layers, int
 double orc_objective(const double* x, int L, const orc_layer* layers, const double target[3],
                      const double* target_alphas, const double* lambda, double rho,
                      double* grad, double* C);
+
+/* The same with the problem variants (v may be NULL: none).  Constraints:
+ * colour rows, alpha rows, then sqrt(3) c - |c| 1 for each gray layer in
+ * layer order (equations.cpp:144-152); the Jacobian's gray block
+ * sqrt(3) I - c 1^T / |c|, zero when |c| <= 1e-3 (:385-398); the energy's
+ * sparsity term sigma (sum alpha / sum alpha^2 - 1) (:104-105). */
+int orc_gray_count(const orc_variant* v, int L);
+double orc_energy_ex(const double* x, int L, const orc_layer* layers, int minimum_alpha, const orc_variant* v);
+int orc_constraints_ex(const double* x, int L, const orc_layer* layers, const double target[3],
+                       const double* target_alphas, const orc_variant* v, double* C);
+int orc_constraint_jacobian_ex(const double* x, int L, const orc_layer* layers, int refine, const orc_variant* v,
+                               double* J);
+double orc_objective_ex(const double* x, int L, const orc_layer* layers, const double target[3],
+                        const double* target_alphas, const double* lambda, double rho, const orc_variant* v,
+                        double* grad, double* C);
 
 /* ---- per-pixel solve ---- */
 /* Solves one pixel.  init: NULL for the default first-pass initialisation,

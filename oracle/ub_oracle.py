@@ -43,7 +43,14 @@ class OrcSettings(ctypes.Structure):
         ("beta", ctypes.c_double),
         ("initial_rho", ctypes.c_double),
         ("alpha_fixed", ctypes.c_int32),
+        ("gray_mask", ctypes.c_int32),
+        ("sparsity", ctypes.c_int32),
+        ("sigma", ctypes.c_double),
     ]
+
+
+class OrcVariant(ctypes.Structure):
+    _fields_ = [("gray_mask", ctypes.c_int32), ("sparsity", ctypes.c_int32), ("sigma", ctypes.c_double)]
 
 
 class OrcCounters(ctypes.Structure):
@@ -96,6 +103,15 @@ def lib():
         L.orc_objective.restype = ctypes.c_double
         L.orc_objective.argtypes = [dp, ctypes.c_int, lp, dp, dp, dp, ctypes.c_double, dp, dp]
         L.orc_default_settings.argtypes = [ctypes.POINTER(OrcSettings)]
+        vp = ctypes.POINTER(OrcVariant)
+        L.orc_energy_ex.restype = ctypes.c_double
+        L.orc_energy_ex.argtypes = [dp, ctypes.c_int, lp, ctypes.c_int, vp]
+        L.orc_constraints_ex.restype = ctypes.c_int
+        L.orc_constraints_ex.argtypes = [dp, ctypes.c_int, lp, dp, dp, vp, dp]
+        L.orc_constraint_jacobian_ex.restype = ctypes.c_int
+        L.orc_constraint_jacobian_ex.argtypes = [dp, ctypes.c_int, lp, ctypes.c_int, vp, dp]
+        L.orc_objective_ex.restype = ctypes.c_double
+        L.orc_objective_ex.argtypes = [dp, ctypes.c_int, lp, dp, dp, dp, ctypes.c_double, vp, dp, dp]
         L.orc_tree_dot.restype = ctypes.c_double
         L.orc_tree_dot.argtypes = [dp, dp, ctypes.c_int]
         L.orc_lbfgs_direction.argtypes = [dp, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, dp]
@@ -180,6 +196,49 @@ def default_settings(**overrides) -> OrcSettings:
     return s
 
 
+def variant(gray_mask=0, sparsity=0, sigma=10.0):
+    v = OrcVariant()
+    v.gray_mask, v.sparsity, v.sigma = int(gray_mask), int(sparsity), float(sigma)
+    return v
+
+
+def energy_ex(x, layers, minimum_alpha, v):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_energy_ex(_dp(x), len(layers), layer_array(layers), int(minimum_alpha), ctypes.byref(v))
+
+
+def constraints_ex(x, layers, target, target_alphas, v):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    C = np.zeros(3 + 4 * MAX_LAYERS)
+    ta = None if target_alphas is None else np.ascontiguousarray(target_alphas, dtype=np.float64)
+    nc = lib().orc_constraints_ex(_dp(x), len(layers), layer_array(layers), _dp(t), None if ta is None else _dp(ta),
+                                  ctypes.byref(v), _dp(C))
+    return C[:nc]
+
+
+def constraint_jacobian_ex(x, layers, refine, v):
+    L = len(layers)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nc = (3 + L if refine else 4) + 3 * bin(v.gray_mask & ((1 << L) - 1)).count("1")
+    J = np.zeros(4 * L * nc)
+    lib().orc_constraint_jacobian_ex(_dp(x), L, layer_array(layers), int(bool(refine)), ctypes.byref(v), _dp(J))
+    return J.reshape(4 * L, nc)
+
+
+def objective_ex(x, layers, target, target_alphas, lam, rho, v):
+    L = len(layers)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    ta = None if target_alphas is None else np.ascontiguousarray(target_alphas, dtype=np.float64)
+    g = np.zeros(4 * L)
+    C = np.zeros(3 + 4 * MAX_LAYERS)
+    f = lib().orc_objective_ex(_dp(x), L, layer_array(layers), _dp(t), None if ta is None else _dp(ta), _dp(lam),
+                               float(rho), ctypes.byref(v), _dp(g), _dp(C))
+    return f, g
+
+
 def blend(s, d, mode):
     return lib().orc_blend(float(s), float(d), int(mode))
 
@@ -207,7 +266,7 @@ def energy(x, layers, minimum_alpha):
 def constraints(x, layers, target, target_alphas=None):
     x = np.ascontiguousarray(x, dtype=np.float64)
     t = np.ascontiguousarray(target, dtype=np.float64)
-    C = np.zeros(3 + MAX_LAYERS)
+    C = np.zeros(3 + 4 * MAX_LAYERS)
     ta = None if target_alphas is None else np.ascontiguousarray(target_alphas, dtype=np.float64)
     nc = lib().orc_constraints(_dp(x), len(layers), layer_array(layers), _dp(t),
                                None if ta is None else _dp(ta), _dp(C))
@@ -237,7 +296,7 @@ def objective(x, layers, target, target_alphas, lam, rho):
     lam = np.ascontiguousarray(lam, dtype=np.float64)
     ta = None if target_alphas is None else np.ascontiguousarray(target_alphas, dtype=np.float64)
     g = np.zeros(4 * L)
-    C = np.zeros(3 + MAX_LAYERS)
+    C = np.zeros(3 + 4 * MAX_LAYERS)
     f = lib().orc_objective(_dp(x), L, layer_array(layers), _dp(t), None if ta is None else _dp(ta),
                             _dp(lam), float(rho), _dp(g), _dp(C))
     return f, g

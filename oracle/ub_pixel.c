@@ -171,11 +171,24 @@ static void distance_gradient(const double c[3], const orc_layer* layer, double 
         g[j] = (2.0 * P[3 * j + 0]) * d0 + (2.0 * P[3 * j + 1]) * d1 + (2.0 * P[3 * j + 2]) * d2;
 }
 
-/* equations.cpp:88-111 (sparsity off: unblending.cpp:17,236-240). */
-double orc_energy(const double* x, int L, const orc_layer* layers, int minimum_alpha)
+/* equations.cpp:104-105: sigma (sum alpha / sum alpha^2 - 1), the sums in
+ * index order. */
+static double sparsity_term(const double* alphas, int L, double sigma)
+{
+    double sum = 0.0, squared = 0.0;
+    for (int i = 0; i < L; ++i) sum += alphas[i];
+    for (int i = 0; i < L; ++i) squared += alphas[i] * alphas[i];
+    return sigma * ((sum / squared) - 1.0);
+}
+
+/* equations.cpp:88-111: main term, the sparsity term when it is on (the
+ * reference's SPARCITY build, first pass only: unblending.cpp:17,236-240),
+ * the minimum-alpha term. */
+double orc_energy_ex(const double* x, int L, const orc_layer* layers, int minimum_alpha, const orc_variant* v)
 {
     double e = 0.0;
     for (int i = 0; i < L; ++i) e += x[i] * orc_distance(x + L + 3 * i, &layers[i]);
+    if (v && v->sparsity) e += sparsity_term(x, L, v->sigma);
     if (minimum_alpha) {
         double sum = 0.0;
         for (int i = 0; i < L; ++i) sum += x[i];
@@ -184,19 +197,57 @@ double orc_energy(const double* x, int L, const orc_layer* layers, int minimum_a
     return e;
 }
 
-/* equations.cpp:113-155 (no gray layers: unblending.cpp:168-172). */
-int orc_constraints(const double* x, int L, const orc_layer* layers, const double target[3],
-                    const double* target_alphas, double* C)
+double orc_energy(const double* x, int L, const orc_layer* layers, int minimum_alpha)
+{
+    return orc_energy_ex(x, L, layers, minimum_alpha, NULL);
+}
+
+/* equations.cpp:144-152: gray rows sqrt(3) c - |c| 1 of the gray layer's
+ * colour c, |c| = sqrt(c0 c0 + c1 c1 + c2 c2). */
+static void gray_rows(const double* c, double* C)
+{
+    const double norm = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int j = 0; j < 3; ++j) C[j] = ORC_SQRT3 * c[j] - norm * 1.0;
+}
+
+int orc_gray_count(const orc_variant* v, int L)
+{
+    int n = 0;
+    if (v)
+        for (int l = 0; l < L; ++l) n += (v->gray_mask >> l) & 1;
+    return n;
+}
+
+/* equations.cpp:113-155: colour rows, alpha rows, then 3 rows per gray layer
+ * in layer order (the reference's SECOND_LAYER_GRAY build makes layer 1
+ * gray, unblending.cpp:168-172). */
+int orc_constraints_ex(const double* x, int L, const orc_layer* layers, const double target[3],
+                       const double* target_alphas, const orc_variant* v, double* C)
 {
     double comp[4];
     orc_composite_stack(x, L, layers, comp);
     for (int c = 0; c < 3; ++c) C[c] = comp[c] - target[c];
+    int nc;
     if (target_alphas) {
         for (int i = 0; i < L; ++i) C[3 + i] = x[i] - target_alphas[i];
-        return 3 + L;
+        nc = 3 + L;
+    } else {
+        C[3] = comp[3] - 1.0;
+        nc = 4;
     }
-    C[3] = comp[3] - 1.0;
-    return 4;
+    if (v)
+        for (int l = 0; l < L; ++l)
+            if ((v->gray_mask >> l) & 1) {
+                gray_rows(x + L + 3 * l, C + nc);
+                nc += 3;
+            }
+    return nc;
+}
+
+int orc_constraints(const double* x, int L, const orc_layer* layers, const double target[3],
+                    const double* target_alphas, double* C)
+{
+    return orc_constraints_ex(x, L, layers, target, target_alphas, NULL, C);
 }
 
 /* equations.cpp:157-196, including the first-pass assignment (not addition)
@@ -316,6 +367,52 @@ static mat4 d_k_by_i(const double* x, int L, const orc_layer* layers, int i, int
     return mat4_mul(&d1, &d2);
 }
 
+/* equations.cpp:385-398: the gray rows' block sqrt(3) I - (1 / |c|) [c c c]
+ * (entry (i, j) = sqrt(3) [i == j] - c_i / |c|), zero when |c| <= 1e-3. */
+static void gray_block(const double* c, double B[3][3])
+{
+    const double norm = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (!(norm > 1e-3)) {
+        memset(B, 0, sizeof(double) * 9);
+        return;
+    }
+    const double inv = 1.0 / norm;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B[i][j] = ORC_SQRT3 * (i == j ? 1.0 : 0.0) - inv * c[i];
+}
+
+int orc_constraint_jacobian_ex(const double* x, int L, const orc_layer* layers, int refine, const orc_variant* v,
+                               double* J)
+{
+    const int base = refine ? 3 + L : 4;
+    const int nc = base + 3 * orc_gray_count(v, L);
+    memset(J, 0, sizeof(double) * (size_t)(4 * L) * (size_t)nc);
+    for (int i = 0; i < L; ++i) {
+        const mat4 d = d_k_by_i(x, L, layers, i, L - 1);
+        if (refine) {
+            for (int c = 0; c < 3; ++c) J[i * nc + c] = d.m[3][c];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) J[(L + 3 * i + r) * nc + c] = d.m[r][c];
+            J[i * nc + 3 + i] = 1.0;
+        } else {
+            for (int c = 0; c < 4; ++c) J[i * nc + c] = d.m[3][c];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 4; ++c) J[(L + 3 * i + r) * nc + c] = d.m[r][c];
+        }
+    }
+    int q = base;
+    if (v)
+        for (int l = 0; l < L; ++l)
+            if ((v->gray_mask >> l) & 1) {
+                double B[3][3];
+                gray_block(x + L + 3 * l, B);
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) J[(L + 3 * l + r) * nc + q + c] = B[r][c];
+                q += 3;
+            }
+    return nc;
+}
+
 /* equations.cpp:335-401 */
 int orc_constraint_jacobian(const double* x, int L, const orc_layer* layers, int refine, double* J)
 {
@@ -345,6 +442,19 @@ int orc_constraint_jacobian(const double* x, int L, const orc_layer* layers, int
 double orc_objective(const double* x, int L, const orc_layer* layers, const double target[3],
                      const double* target_alphas, const double* lambda, double rho,
                      double* grad, double* C)
+{
+    return orc_objective_ex(x, L, layers, target, target_alphas, lambda, rho, NULL, grad, C);
+}
+
+/* With the variant's terms: the sparsity term in the energy (the reference
+ * turns it on in the first pass only, unblending.cpp:236-240, where the
+ * alpha block's overwrite with 0.01 discards its gradient,
+ * equations.cpp:177-193), and the gray rows in C with their part of
+ * J (rho C - lambda), sum_j B[i][j] w_j in j order, added to the gray
+ * colour's gradient after the sweep. */
+double orc_objective_ex(const double* x, int L, const orc_layer* layers, const double target[3],
+                        const double* target_alphas, const double* lambda, double rho, const orc_variant* v,
+                        double* grad, double* C)
 {
     const int refine = target_alphas != NULL;
     const double* alphas = x;
@@ -383,6 +493,14 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
         C[3] = pre[L - 1][3] - 1.0;
         nc = 4;
     }
+    const int nbase = nc;
+    if (v)
+        for (int l = 0; l < L; ++l)
+            if ((v->gray_mask >> l) & 1) {
+                gray_rows(colors + 3 * l, C + nc);
+                nc += 3;
+            }
+    if (v && v->sparsity) energy += sparsity_term(alphas, L, v->sigma);
     if (!refine) {
         double sum = 0.0;
         for (int i = 0; i < L; ++i) sum += alphas[i];
@@ -440,5 +558,18 @@ double orc_objective(const double* x, int L, const orc_layer* layers, const doub
     if (refine) ga0 = ga0 + w[3];
     grad[0] = (refine ? dist[0] : 0.01) + ga0;
     for (int c = 0; c < 3; ++c) grad[L + c] = grad[L + c] + aB[c];
+    int q = nbase;
+    if (v)
+        for (int l = 0; l < L; ++l)
+            if ((v->gray_mask >> l) & 1) {
+                double B[3][3];
+                gray_block(colors + 3 * l, B);
+                for (int i = 0; i < 3; ++i) {
+                    double t = 0.0;
+                    for (int j = 0; j < 3; ++j) t = t + B[i][j] * w[q + j];
+                    grad[L + 3 * l + i] = grad[L + 3 * l + i] + t;
+                }
+                q += 3;
+            }
     return f;
 }

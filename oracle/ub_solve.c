@@ -59,6 +59,9 @@ void orc_default_settings(orc_settings* s)
     s->beta = 10.0;        /* :223 */
     s->initial_rho = 100.0;/* :224 */
     s->alpha_fixed = 0;    /* MATTE_ALPHA_FORCE_FIXED off, unblending.cpp:19 */
+    s->gray_mask = 0;      /* SECOND_LAYER_GRAY off, unblending.cpp:21 */
+    s->sparsity = 0;       /* SPARCITY off, unblending.cpp:17 */
+    s->sigma = 10.0;       /* unblending.cpp:234 */
 }
 
 static double clampd(double v, double lo, double hi)
@@ -311,13 +314,14 @@ typedef struct {
     const double* target_alphas;
     double lambda[ORC_MAX_CONSTRAINTS];
     double rho;
+    orc_variant v;
 } pixel_problem;
 
 static double pixel_fn(const double* x, double* grad, void* data)
 {
     const pixel_problem* p = (const pixel_problem*)data;
     double C[ORC_MAX_CONSTRAINTS];
-    return orc_objective(x, p->L, p->layers, p->target, p->target_alphas, p->lambda, p->rho, grad, C);
+    return orc_objective_ex(x, p->L, p->layers, p->target, p->target_alphas, p->lambda, p->rho, &p->v, grad, C);
 }
 
 static double norm(const double* v, int n)
@@ -392,7 +396,11 @@ static int solve_pixel_from(const double target[3], int L, const orc_layer* laye
     p.layers = layers;
     p.target = target;
     p.target_alphas = target_alphas;
-    const int nc = refine ? 3 + L : 4;
+    /* unblending.cpp:236-243: sparsity in the first pass only */
+    p.v.gray_mask = st->gray_mask;
+    p.v.sparsity = refine ? 0 : st->sparsity;
+    p.v.sigma = st->sigma;
+    const int nc = (refine ? 3 + L : 4) + 3 * orc_gray_count(&p.v, L);
     for (int j = 0; j < nc; ++j) p.lambda[j] = g_lambda_init ? g_lambda_init[j] : 0.0;
     p.rho = st->initial_rho;
 
@@ -410,8 +418,8 @@ static int solve_pixel_from(const double target[3], int L, const orc_layer* laye
             if (pairs > counters->max_pairs) counters->max_pairs = pairs;
         }
         double g[ORC_MAX_CONSTRAINTS], gn[ORC_MAX_CONSTRAINTS], dx[ORC_MAX_VARS];
-        orc_constraints(x, L, layers, target, target_alphas, g);
-        orc_constraints(xn, L, layers, target, target_alphas, gn);
+        orc_constraints_ex(x, L, layers, target, target_alphas, &p.v, g);
+        orc_constraints_ex(xn, L, layers, target, target_alphas, &p.v, gn);
         for (int j = 0; j < nc; ++j) p.lambda[j] = p.lambda[j] - p.rho * gn[j];
         const double ngn = norm(gn, nc);
         if (ngn > st->gamma * norm(g, nc)) p.rho = p.rho * st->beta;

@@ -152,7 +152,7 @@ class HipParams(ctypes.Structure):
                 ("solver", ctypes.c_int32), ("timing", ctypes.c_int32), ("warm_start", ctypes.c_int32),
                 ("mlp", ctypes.c_void_p), ("chunk_size", ctypes.c_int32), ("filter_block", ctypes.c_int32),
                 ("filter_rows", ctypes.c_int32), ("warm_outer_cap", ctypes.c_int32),
-                ("lbfgs_memory", ctypes.c_int32)]
+                ("lbfgs_memory", ctypes.c_int32), ("gray_layers", ctypes.c_int32), ("sparsity", ctypes.c_int32)]
 
 
 class HipImage(ctypes.Structure):
@@ -309,12 +309,15 @@ def options(backend=BACKEND_AUTO, opaque=True, smooth=False, target_concurrency=
 
 
 def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, warm_start=None, chunk_size=0,
-               filter_block=0, filter_rows=0, warm_outer_cap=0, lbfgs_memory=0) -> HipParams:
+               filter_block=0, filter_rows=0, warm_outer_cap=0, lbfgs_memory=0, gray_layers=0,
+               sparsity=False) -> HipParams:
     """warm_start: a trained ``mlp.WarmStartModel`` (first pass only), or None.
     chunk_size / filter_block / filter_rows: launch shapes (0: defaults; see
     ``unblending_amd.autotune``).  lbfgs_memory: (s, y) pairs of the inner
     minimiser; 0 = NLopt's default for LD_LBFGS (every pair of an inner
-    solve), 1 = the one-pair fast mode."""
+    solve), 1 = the one-pair fast mode.  gray_layers: bit mask of layers whose
+    colour is constrained to gray (the reference's SECOND_LAYER_GRAY: 0b10);
+    sparsity: the first pass's sparsity term (the reference's SPARCITY)."""
     p = HipParams()
     p.struct_size = ctypes.sizeof(HipParams)
     p.filter_radius = int(radius)
@@ -329,6 +332,8 @@ def hip_params(radius=-1, eps=0.0, alpha_fixed=False, solver=0, timing=False, wa
     p.filter_rows = int(filter_rows)
     p.warm_outer_cap = int(warm_outer_cap)
     p.lbfgs_memory = int(lbfgs_memory)
+    p.gray_layers = int(gray_layers)
+    p.sparsity = int(bool(sparsity))
     return p
 
 

@@ -231,12 +231,25 @@ __device__ __forceinline__ const DevLayer& layer_after(const DevLayer* lay, int 
     return lay[k + zero];
 }
 
-// Number of constraints: 4 in the first pass, 3 + L in refinement.
-template <int L, bool REFINE>
+// Number of constraints: 4 in the first pass, 3 + L in refinement, and 3
+// more for a gray layer (GRAY: layer 1, the reference's SECOND_LAYER_GRAY
+// build, unblending.cpp:168-172).
+template <int L, bool REFINE, bool GRAY = false>
 struct Shape {
     static constexpr int N = 4 * L;
-    static constexpr int NC = REFINE ? 3 + L : 4;
+    static constexpr int NB = REFINE ? 3 + L : 4;  // colour and alpha rows
+    static constexpr int NC = NB + (GRAY ? 3 : 0);
 };
+
+constexpr double kSqrt3 = 1.7320508075688772;  // std::sqrt(3), correctly rounded
+
+// The gray rows of a colour c (equations.cpp:144-152): sqrt(3) c - |c| 1.
+__device__ __forceinline__ void gray_rows(const double* c, double* C)
+{
+    const double norm = __builtin_sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[j] = kSqrt3 * c[j] - norm * 1.0;
+}
 
 // Augmented-Lagrangian objective and its gradient at x.
 // x = [alpha_0..alpha_{L-1}, c_0 rgb .. c_{L-1} rgb].  The forward sweep
@@ -252,12 +265,21 @@ struct Shape {
 // constraints that depend on the composite: the colour residual (3) and, in
 // the first pass, the alpha residual (the refinement's alpha constraints
 // alpha_i - t_i need no composite).
-template <int L, bool REFINE, class TG, class TA, class TL>
+//
+// Problem variants: GRAY adds layer 1's gray rows to C and their part of
+// J (rho C - lambda) -- (sqrt(3) I - c 1^T / |c|) w, zero when |c| <= 1e-3
+// (equations.cpp:385-398), each row summed in column order -- to that
+// colour's gradient after the sweep; SPARSE adds the sparsity term
+// sparse_sigma (sum a / sum a^2 - 1) to the energy (equations.cpp:104-105;
+// first pass only, where the alpha block's overwrite discards its gradient).
+template <int L, bool REFINE, bool GRAY = false, bool SPARSE = false, class TG, class TA, class TL>
 __device__ __forceinline__ double objective(const double* __restrict__ x, const DevLayer* lay, const TG& tgt,
                                             const TA& ta, const TL& lam, const double rho,
-                                            double* __restrict__ grad, double* __restrict__ Cc)
+                                            double* __restrict__ grad, double* __restrict__ Cc,
+                                            const double sparse_sigma = 0.0)
 {
-    constexpr int NC = Shape<L, REFINE>::NC;
+    constexpr int NC = Shape<L, REFINE, GRAY>::NC;
+    constexpr int NB = Shape<L, REFINE, GRAY>::NB;
     const double* alphas = x;
     const double* colors = x + L;
 
@@ -297,6 +319,7 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
     } else {
         C[3] = pre[L - 1][3] - 1.0;
     }
+    if constexpr (GRAY) gray_rows(colors + 3, C + NB);
 #pragma unroll
     for (int j = 0; j < (REFINE ? 3 : 4); ++j) Cc[j] = C[j];
     double lagrange = 0.0, squared = 0.0;
@@ -307,8 +330,9 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
 
     // rho C - lambda: the composite rows here; the refinement's alpha rows
     // (rho (alpha_i - t_i) - lambda_{3+i}, the same expression) where the
-    // reverse sweep uses them, so that they are not live across it
-    constexpr int NW = REFINE ? 3 : NC;
+    // reverse sweep uses them, and the gray rows after it, so that they are
+    // not live across it
+    constexpr int NW = REFINE ? 3 : 4;
     double w[NW];
 #pragma unroll
     for (int j = 0; j < NW; ++j) w[j] = rho * C[j] - lam[j];
@@ -373,10 +397,35 @@ __device__ __forceinline__ double objective(const double* __restrict__ x, const 
 #pragma unroll
         for (int c = 0; c < 3; ++c) grad[L + c] = alphas[0] * gE[c] + aB[c];
     }
+    if constexpr (GRAY) {
+        const double* c = colors + 3;
+        const double norm = __builtin_sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        if (norm > 1e-3) {
+            const double inv = 1.0 / norm;
+            double wg[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) wg[j] = rho * C[NB + j] - lam[NB + j];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) t = t + (kSqrt3 * (i == j ? 1.0 : 0.0) - inv * c[i]) * wg[j];
+                grad[L + 3 + i] = grad[L + 3 + i] + t;
+            }
+        }
+    }
     // the energy, its terms summed bottom to top as the reference does
     double energy = 0.0;
 #pragma unroll
     for (int k = 0; k < L; ++k) energy += alphas[k] * dist[k];
+    if constexpr (SPARSE && !REFINE) {
+        double sum = 0.0, squared = 0.0;
+#pragma unroll
+        for (int i = 0; i < L; ++i) sum += alphas[i];
+#pragma unroll
+        for (int i = 0; i < L; ++i) squared += alphas[i] * alphas[i];
+        energy += sparse_sigma * ((sum / squared) - 1.0);
+    }
     if constexpr (!REFINE) {
         double sum = 0.0;
 #pragma unroll

@@ -54,6 +54,8 @@ struct DeviceJob {
 // it): max(1310720 / n, 10), lowered to max_evals when max_evals is not above
 // that (NLopt src/algs/luksan/plis.c) -- 1000 at the reference's maxeval.
 int lbfgs_memory(int n, int max_evals, int memory);
+// Throws std::invalid_argument for problem variants the device does not solve.
+void check_variants(const unblending_hip_params& p, int layers);
 constexpr int kMaxEvals = 1000;  // unblending.cpp:249
 
 // Copies a caller's params (struct_size bytes of them, at least up to

@@ -32,7 +32,9 @@ struct SolverParams {
     int alpha_fixed; // refinement alphas fixed by bounds instead of constraints
     int warm_max_count;  // warm-started attempt: outer cap before the pixel is solved from the default start
     int memory;      // (s, y) pairs kept: 1 = the register-resident minimiser, else a history in the workspace
+    int gray;        // 1: layer 1's colour constrained to gray (history-keeping kernel, 2 to 8 layers)
     double ftol, xtol, eps, gamma, beta, rho0;
+    double sparse_sigma;  // the sparsity term's weight in the energy, 0 = off (first pass only)
 };
 
 // Per-pass work counters accumulated by the solve kernels.
@@ -112,6 +114,9 @@ constexpr int kDefaultChunk = 64;  // pixels a wave takes from the queue at once
 
 // Every image of the batch must have L layers.
 hipError_t launch_batch_solve(int L, bool refine, const BatchArgs& args, hipStream_t stream);
+// The same for the problem variants (args.sp.gray, args.sp.sparse_sigma):
+// history-keeping kernels only.
+hipError_t launch_batch_variant(int L, bool refine, const BatchArgs& args, hipStream_t stream);
 
 // One filtered channel: value(pixel) = base[pixel * stride].
 struct Channel {

@@ -48,9 +48,11 @@ DevLayer to_device(const unblending_layer& l)
 
 constexpr int kDefaultWarmOuterCap = 8;
 
-SolverParams default_solver(const unblending_hip_params& p, int layers)
+SolverParams default_solver(const unblending_hip_params& p, int layers, bool refine)
 {
     SolverParams s;
+    s.gray = p.gray_layers != 0 ? 1 : 0;
+    s.sparse_sigma = (p.sparsity && !refine) ? 10.0 : 0.0;  // unblending.cpp:234-240
     s.warm_max_count = p.warm_outer_cap > 0 ? p.warm_outer_cap : kDefaultWarmOuterCap;
     s.max_evals = kMaxEvals;  // unblending.cpp:249
     s.memory = lbfgs_memory(4 * layers, s.max_evals, p.lbfgs_memory);
@@ -140,6 +142,17 @@ size_t history_bytes(long long chunks, int layers, int memory)
 long long default_chunks(size_t n) { return (long long)((n + kDefaultChunk - 1) / kDefaultChunk); }
 
 }  // namespace
+
+void check_variants(const unblending_hip_params& p, int layers)
+{
+    if (p.sparsity != 0 && p.sparsity != 1) throw std::invalid_argument("sparsity must be 0 or 1");
+    if (p.gray_layers == 0) return;
+    if (p.gray_layers != 2)
+        throw std::invalid_argument("The device solves the second-layer gray variant only (gray_layers = 2)");
+    if (layers < 2) throw std::invalid_argument("A gray second layer needs at least 2 layers");
+    if (lbfgs_memory(4 * layers, kMaxEvals, p.lbfgs_memory) == 1)
+        throw std::invalid_argument("The gray variant runs with the default L-BFGS history (lbfgs_memory != 1)");
+}
 
 int lbfgs_memory(int n, int max_evals, int memory)
 {
@@ -309,7 +322,7 @@ void run_device(const DeviceJob& job)
     for (size_t l = 0; l < L; ++l) args.layers[l] = to_device(job.set->host[l]);
     args.rgb = job.rgb;
     args.n = (long long)n;
-    args.sp = default_solver(job.params, int(L));
+    args.sp = default_solver(job.params, int(L), refine);
     args.opaque = job.opaque ? 1 : 0;
     args.smooth = job.smooth ? 1 : 0;
     args.all_plus = job.set->all_plus ? 1 : 0;
@@ -360,6 +373,9 @@ void run_device(const DeviceJob& job)
     if (warm && args.solver != 0) throw std::invalid_argument("The warm start runs on the persistent solve kernel");
     if (args.sp.memory != 1 && args.solver != 0)
         throw std::invalid_argument("The one-pixel-per-lane kernel keeps one (s, y) pair (lbfgs_memory = 1)");
+    check_variants(job.params, int(L));
+    if (args.solver != 0 && (job.params.gray_layers || job.params.sparsity))
+        throw std::invalid_argument("The problem variants run on the persistent solve kernel");
     ImageDesc desc{};
     BatchArgs batch{};
     ImageDesc* ddesc = reinterpret_cast<ImageDesc*>(ws + kDescOffset);
@@ -527,6 +543,7 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return sets[a].host.size() < sets[b].host.size(); });
     const int chunk = params.chunk_size > 0 ? params.chunk_size : kDefaultChunk;
+    for (size_t i = 0; i < count; ++i) check_variants(params, int(sets[i].host.size()));
     struct Group {
         int L;
         size_t first, count;
@@ -577,7 +594,7 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
         a.image_count = int(groups[g].count);
         a.chunk_size = chunk;
         a.chunk_count = groups[g].chunks;
-        a.sp = default_solver(params, groups[g].L);
+        a.sp = default_solver(params, groups[g].L, refine);
         a.lane_state = reinterpret_cast<double*>(ws + lay.lane_state);
         a.counters = counters;
         a.queue = kGroupQueue + int(g);
