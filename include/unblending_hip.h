@@ -193,7 +193,10 @@ UNBLENDING_API int32_t unblending_hip_mlp_predict(
  * the per-pixel results are those of unblending_hip_unmix_device /
  * unblending_hip_refine_device on each image alone.  The options apply to
  * every image.  Enqueued on `stream`; no allocation, no synchronisation
- * unless stats are asked for, so a call can be captured in a HIP graph. */
+ * unless stats are asked for, so a call can be captured in a HIP graph.
+ * With params.warm_start = 1, the first pass starts each image that has a
+ * model (unblending_hip_image.mlp) from that model's prediction, as
+ * unblending_hip_unmix_device does for one image (params.mlp is not used). */
 typedef struct unblending_hip_image {
     const double* rgb;               /* device [height][width][3] */
     int32_t width, height;
@@ -205,6 +208,9 @@ typedef struct unblending_hip_image {
     double* output;                  /* device packed [layer][height][width][RGBA] */
     int32_t filter_radius;           /* refinement: < 0: 60 * min(width, height) / 1000 */
     int32_t reserved;
+    const void* mlp;                 /* first pass with params.warm_start = 1: the trained model of this
+                                        image's layer set (unblending_hip_mlp_*), or NULL for the
+                                        default initialisation */
 } unblending_hip_image;
 
 UNBLENDING_API size_t unblending_hip_batch_workspace_size(const unblending_hip_image* images, size_t count,

@@ -255,9 +255,8 @@ def test_batch_entry_points_validate():
     assert unmix(None, 3, ctypes.byref(opt), None, buf, ws1, None, None) == U.INVALID_ARGUMENT
     assert unmix(imgs, 0, ctypes.byref(opt), None, buf, ws1, None, None) == U.INVALID_ARGUMENT
     assert unmix(imgs, 3, ctypes.byref(opt), None, buf, ws1 - 1, None, None) == U.BUFFER_TOO_SMALL
-    warm = U.hip_params()
-    warm.warm_start = 1
-    assert unmix(imgs, 3, ctypes.byref(opt), ctypes.byref(warm), buf, ws1, None, None) == U.INVALID_ARGUMENT
+    negative = U.hip_params(lbfgs_memory=-1)
+    assert unmix(imgs, 3, ctypes.byref(opt), ctypes.byref(negative), buf, ws1, None, None) == U.INVALID_ARGUMENT
     simple = U.hip_params(solver=1)
     assert unmix(imgs, 3, ctypes.byref(opt), ctypes.byref(simple), buf, ws1, None, None) == U.INVALID_ARGUMENT
     # image checks in image order: a bad layer set, a missing first pass, too many layers

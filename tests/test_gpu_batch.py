@@ -136,3 +136,55 @@ def test_batch_graph_replays_match_eager():
         # eager work between replays
         device.unmix(other, layer_infos(wls[0].layers))
         torch.cuda.synchronize()
+
+
+def test_warm_started_batch_matches_single_images(oracle):
+    """Per-set warm start in a batch: each image with a model of its own layer
+    set starts from that model's prediction; an image without one from the
+    default initialisation.  Every image's first pass equals the
+    single-image warm (or plain) call bit for bit, the fallback counts add
+    up, and a warm image equals the oracle's warm solve from the same
+    prediction."""
+    import torch
+    import unblending_amd as U
+    from unblending_amd import device, mlp, synth
+    wls = _images(3, 0.06, seed0=30)
+    print("case warm batch: " + ", ".join(wl.name for wl in wls), flush=True)
+    models = []
+    for j, wl in enumerate(wls[:2]):
+        tw = [synth.config5_image(30 + j, 0.06, image_seed=900 + 7 * j + t) for t in range(2)]
+        images = torch.stack([_dev(t.rgb) for t in tw]).contiguous()
+        targets = torch.stack([device.unmix(images[t], layer_infos(wl.layers)) for t in range(2)]).contiguous()
+        m = mlp.WarmStartModel(len(wl.layers), seed=5 + j)
+        m.fit(images, targets, steps=300, batch=1024, lr=3e-3, chunk=100)
+        models.append(m)
+    imgs = _batch(wls)
+    imgs[0].model, imgs[1].model = models
+    st = U.HipStats()
+    device.unmix_batch(imgs, params=U.hip_params(warm_start=models[0]), stats=st)
+    torch.cuda.synchronize()
+    fallback = 0
+    for j, (wl, im) in enumerate(zip(wls, imgs)):
+        if j < 2:
+            s1 = U.HipStats()
+            single = device.unmix(im.rgb, im.layer_infos, params=U.hip_params(warm_start=models[j]), stats=s1)
+            fallback += s1.fallback_pixels
+        else:
+            single = device.unmix(im.rgb, im.layer_infos)
+        assert torch.equal(single, im.out), wl.name
+    assert st.fallback_pixels == fallback
+    # one warm image against the oracle's warm solve from the same prediction
+    wl, im = wls[0], imgs[0]
+    L = len(wl.layers)
+    pred = models[0].predict(im.rgb).cpu().numpy()
+    ol = oracle_layers(wl.layers)
+    out = im.out.cpu().numpy()
+    warm_settings = oracle.default_settings()
+    warm_settings.max_count = 8
+    for (y, x) in [(0, 0), (wl.height // 2, wl.width // 3), (wl.height - 1, wl.width - 1)]:
+        x0 = np.concatenate([pred[:, y, x, 3], pred[:, y, x, :3].reshape(-1)])
+        sol, _, cap = oracle.solve_pixel_warm(wl.rgb[y, x], ol, x0, settings=warm_settings)
+        if cap:
+            sol, _ = oracle.solve_pixel(wl.rgb[y, x], ol)
+        assert np.array_equal(out[:, y, x, 3], sol[:L])
+        assert np.array_equal(out[:, y, x, :3], sol[L:].reshape(L, 3))

@@ -160,7 +160,7 @@ class HipImage(ctypes.Structure):
     _fields_ = [("rgb", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
                 ("layers", ctypes.c_void_p), ("layer_count", ctypes.c_size_t), ("first_pass", ctypes.c_void_p),
                 ("alpha_planes", ctypes.c_void_p), ("output", ctypes.c_void_p), ("filter_radius", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("mlp", ctypes.c_void_p)]
 
 
 class HipStats(ctypes.Structure):

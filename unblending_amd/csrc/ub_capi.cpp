@@ -278,8 +278,7 @@ int32_t batch_call(const unblending_hip_image* images, size_t count, const unble
             return UNBLENDING_INVALID_ARGUMENT;
         unblending_hip_params p;
         if (!read_params(params, p)) return UNBLENDING_INVALID_ARGUMENT;
-        if (p.warm_start != 0 || p.solver != 0 || p.chunk_size < 0 || p.lbfgs_memory < 0)
-            return UNBLENDING_INVALID_ARGUMENT;
+        if (p.solver != 0 || p.chunk_size < 0 || p.lbfgs_memory < 0) return UNBLENDING_INVALID_ARGUMENT;
         const Backend backend = select_backend(options->backend);
         if (backend == Backend::cpu) return UNBLENDING_INVALID_ARGUMENT;
         if (backend == Backend::cuda) return UNBLENDING_DEVICE_ERROR;
@@ -295,6 +294,8 @@ int32_t batch_call(const unblending_hip_image* images, size_t count, const unble
                 return UNBLENDING_INVALID_ARGUMENT;
             sets.push_back(make_layer_set(im.layers, im.layer_count));
             if (refine && !sets.back().all_plus && !sets.back().all_so) return UNBLENDING_INVALID_ARGUMENT;
+            if (!refine && p.warm_start && im.mlp && 4 * im.layer_count > size_t(mlp::kOutPad))
+                throw ExecutionError("The warm-start model predicts at most 8 layers");
             if (im.layer_count > size_t(kMaxLayers))
                 throw ExecutionError("Layer count is outside the device solver's domain (1 to 8 layers)");
         }

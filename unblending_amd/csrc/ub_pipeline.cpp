@@ -503,6 +503,24 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
     if (stats) *stats = unblending_hip_stats{};
     check(launch_zero(counters, kNumCounters, stream), "zero counters");
 
+    // first pass with a warm start: each image with a model gets that model's
+    // prediction in its output buffer, which its solve reads pixel by pixel
+    // before writing the same pixel's solution
+    const bool warm = !refine && params.warm_start != 0;
+    if (warm) {
+        Events ep(timing);
+        ep.start(stream);
+        for (size_t i = 0; i < count; ++i) {
+            const unblending_hip_image& im = images[i];
+            if (!im.mlp) continue;
+            const mlp::Model md = mlp::model_view(const_cast<void*>(im.mlp));
+            check(mlp_predict(md, im.rgb, im.width, im.height, int(4 * sets[i].host.size()), im.output, stream),
+                  "warm-start prediction");
+        }
+        ep.stop(stream);
+        if (stats) stats->predict_ms = ep.ms();
+    }
+
     // refinement: filter every image's alphas (and background) into its planes
     if (refine) {
         double* ab = reinterpret_cast<double*>(ws + lay.ab);
@@ -570,6 +588,8 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
             d.filtered = planes + lay.q_offset[i];
         } else {
             d.alpha_planes = im.alpha_planes;
+            if (warm && im.mlp) d.warm = im.output;
+            d.fallback_count = counters + kFallback;
         }
         d.chunk_base = chunk_base;
         d.L = L;
@@ -617,6 +637,7 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
         stats->pixels = host_counters[kPixels];
         stats->lane_slots = host_counters[kSlots];
         stats->history_pairs = host_counters[kPairs];
+        stats->fallback_pixels = host_counters[kFallback];
         stats->solve_ms = es.ms();
     }
 }

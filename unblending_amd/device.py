@@ -152,9 +152,11 @@ def composite(layers, layer_infos, out=None, stream=None):
 class BatchImage:
     """One image of a batch: rgb (h, w, 3) on the device, its layer infos,
     its output (L, h, w, 4) (allocated when None) and, for refinement, its
-    first pass (L, h, w, 4), optional alpha planes (L, h, w) and radius."""
+    first pass (L, h, w, 4), optional alpha planes (L, h, w) and radius.
+    model: a trained mlp.WarmStartModel of the image's layer set, used by a
+    first pass whose params ask for the warm start."""
 
-    def __init__(self, rgb, layer_infos, out=None, first=None, alpha_planes=None, radius=-1):
+    def __init__(self, rgb, layer_infos, out=None, first=None, alpha_planes=None, radius=-1, model=None):
         torch = _torch()
         h, w, _ = rgb.shape
         L = len(layer_infos)
@@ -168,6 +170,7 @@ class BatchImage:
             _check(alpha_planes, (L, h, w), "alpha_planes")
         self.rgb, self.layer_infos, self.out, self.first, self.alpha_planes = rgb, layer_infos, out, first, alpha_planes
         self.radius = int(radius)
+        self.model = model
         self._layers = layer_array(layer_infos)
 
     def c_struct(self) -> HipImage:
@@ -181,6 +184,7 @@ class BatchImage:
         im.alpha_planes = self.alpha_planes.data_ptr() if self.alpha_planes is not None else None
         im.output = self.out.data_ptr()
         im.filter_radius = self.radius
+        im.mlp = self.model.data_ptr() if self.model is not None else None
         return im
 
 

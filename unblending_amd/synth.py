@@ -246,7 +246,9 @@ C5_MODES = [NORMAL, MULTIPLY, SCREEN, OVERLAY, COLOR_DODGE, COLOR_BURN, HARD_LIG
             DARKEN, LIGHTEN]
 
 
-def config5_image(j, scale=1.0):
+def config5_image(j, scale=1.0, image_seed=None):
+    """Image j of the batch configuration; image_seed: another image of the
+    same size and layer set (training images of a per-set model)."""
     rng = np.random.default_rng(0x5EED0500 + j)
     side = np.exp(rng.uniform(np.log(256), np.log(2048))) * scale
     aspect = rng.uniform(0.5, 2.0)
@@ -255,6 +257,7 @@ def config5_image(j, scale=1.0):
     L = int(rng.integers(3, 8))
     modes = [NORMAL] + [int(m) for m in rng.choice(C5_MODES[1:], L - 1)]
     layers = [spd_model(rng, m, 0.1, 0.9, 0.002, 0.02) for m in modes]
+    rng = _image_rng(rng, image_seed)
     fields = [None] + [blurred_noise(rng, h, w, 16.0 * scale) for _ in modes[1:]]
     rgb = _render(rng, layers, fields, 0.005, h, w)
     r = max(1, min(w, h) // 64)
