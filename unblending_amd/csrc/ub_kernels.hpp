@@ -17,11 +17,10 @@ constexpr long long kLaneStateDoubles = (long long)kMaxPersistentBlocks * 64 * 4
 // 64-lane blocks of a persistent solve that keeps its (s, y) history in the
 // workspace: one wave per SIMD on the 256 CUs of an MI355X.  Each lane owns
 // a ring of min(memory, max_evals) records of kHistRecord(L) doubles: s and
-// y, each zero-padded from 4L to kHistPad(L) (a multiple of 8) components,
-// then 1 / s.y and the two-loop multiplier a.
+// y (4L components each), then 1 / s.y and the two-loop multiplier a.  A
+// record is a multiple of 16 bytes, so s and y start 16-byte aligned.
 constexpr int kHistBlocks = 1024;
-constexpr int kHistPad(int L) { return 8 * ((L + 1) / 2); }
-constexpr int kHistRecord(int L) { return 2 * kHistPad(L) + 2; }
+constexpr int kHistRecord(int L) { return 8 * L + 2; }
 
 // Solver constants, unblending/src/unblending.cpp:220-249, plus the inner
 // minimiser's own (line-search trials).

@@ -644,24 +644,26 @@ constexpr int kDirBudget = 32;
 // long history does not hold up the evaluations of the rest of the wave.
 // The wave serves the pending pixels 16 at a time, longest remaining work
 // first: lane l of quad g (lanes 4g .. 4g + 3) holds components 8l .. 8l + 7
-// of one pending pixel (the records' zero padding makes every component of a
-// record present) and streams that pixel's records -- the k-th oldest at slot
-// (head + k) mod cap of its ring hist + p * cap * R -- kHistBlock records a
-// load batch.  A dot product is each lane's two four-component partial sums,
-// added, then summed over the quad (quad_sum).  A quad's trip counts are
-// uniform over the quad, so its DPP moves run on whole quads.  Returns the
-// lanes whose direction is complete.
+// of one pending pixel and streams that pixel's records -- the k-th oldest at
+// slot (head + k) mod cap of its ring hist + p * cap * R -- kHistBlock
+// records a load batch.  4L is a multiple of 4, so each half (four
+// components) of a lane's share is wholly inside the vector or wholly past
+// its end; a half past the end is not loaded and holds zeros.  A dot product
+// is each lane's two four-component partial sums, added, then summed over
+// the quad (quad_sum).  A quad's trip counts are uniform over the quad, so
+// its DPP moves run on whole quads.  Returns the lanes whose direction is
+// complete.
 template <int L>
 __device__ __forceinline__ unsigned long long direction_service(unsigned long long pend, double* dvcol,
                                                                 const double* hblock, int cap, int count,
                                                                 int head, double gamma, int& dstep)
 {
     constexpr int N = 4 * L;
-    constexpr int NP = kHistPad(L);
     constexpr int R = kHistRecord(L);
     constexpr int U = kHistBlock;
     const int lane = threadIdx.x, grp = lane >> 2, l = lane & 3;
-    const bool live = 8 * l < NP;
+    const bool live = 8 * l < N;       // components 8l .. 8l + 3 exist
+    const bool upper = 8 * l + 8 <= N;  // components 8l + 4 .. 8l + 7 exist
     const bool mine = (pend >> lane) & 1ull;
     // rank among the pending lanes: most remaining steps first
     const int rem = 2 * count - dstep;
@@ -714,13 +716,26 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
                     int slot = hd + k;
                     if (slot >= cap) slot -= cap;
                     const double* rec = ring + (long long)slot * R;
-                    rv[u] = rec[2 * NP];
-                    av[u] = rec[2 * NP + 1];
+                    rv[u] = rec[2 * N];
+                    av[u] = rec[2 * N + 1];
                     if (live) {
 #pragma unroll
-                        for (int c = 0; c < 8; ++c) {
+                        for (int c = 0; c < 4; ++c) {
                             sv[u][c] = rec[8 * l + c];
-                            yv[u][c] = rec[NP + 8 * l + c];
+                            yv[u][c] = rec[N + 8 * l + c];
+                        }
+                    }
+                    if (upper) {
+#pragma unroll
+                        for (int c = 4; c < 8; ++c) {
+                            sv[u][c] = rec[8 * l + c];
+                            yv[u][c] = rec[N + 8 * l + c];
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 4; c < 8; ++c) {
+                            sv[u][c] = 0.0;
+                            yv[u][c] = 0.0;
                         }
                     }
                 }
@@ -745,7 +760,7 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
                         if (l == 0) {
                             int slot = hd + cnt - 1 - (pos + u);
                             if (slot >= cap) slot -= cap;
-                            ring[(long long)slot * R + 2 * NP + 1] = a;
+                            ring[(long long)slot * R + 2 * N + 1] = a;
                         }
                         if (live) {
 #pragma unroll
@@ -1025,17 +1040,16 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
                             slot = head;
                             head = (head + 1 == cap) ? 0 : head + 1;
                         }
-                        constexpr int NP = kHistPad(L);
                         constexpr int R = kHistRecord(L);
                         double* const rec = (&args.hist)[zero] +
                                             ((long long)blockIdx.x * 64 + lane) * (long long)cap * R +
                                             (long long)slot * R;
 #pragma unroll
-                        for (int i = 0; i < NP; ++i) {
-                            rec[i] = (i < N) ? S[i] : 0.0;
-                            rec[NP + i] = (i < N) ? Y[i] : 0.0;
+                        for (int i = 0; i < N; ++i) {
+                            rec[i] = S[i];
+                            rec[N + i] = Y[i];
                         }
-                        rec[2 * NP] = rh;
+                        rec[2 * N] = rh;
                     } else {
                         count = 1;
                     }
