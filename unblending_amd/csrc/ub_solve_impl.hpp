@@ -628,6 +628,9 @@ __device__ __forceinline__ double quad_sum(double v)
     return v;
 }
 
+// Eight zeros: what a lane reads for components past the end of s or y.
+__device__ double kZeros[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
 // Records of one history ring streamed per load batch of the two-loop
 // recursion, and the recursion steps a pending direction advances per trip.
 constexpr int kHistBlock = 3;
@@ -706,38 +709,45 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
             // follow the stores of the first loop in the wave's program order,
             // which the wave's memory operations to one address keep)
             const int lim = (pos < cnt && cnt < end) ? cnt : end;
-            const int take = min(U, lim - pos);
-            const bool back = pos >= cnt;  // second loop, oldest to newest
-            double sv[U][8], yv[U][8], rv[U], av[U];
+            const int take = min(U, lim - pos);  // <= 0 for a quad whose work is done
+            const bool back = pos >= cnt;        // second loop, oldest to newest
+            // Every lane loads U records a batch: record indices are clamped
+            // to the ring (a quad with fewer records left re-reads one), so the
+            // loads are unconditional and their registers carry nothing from
+            // one batch to the next; records past `take` are not computed.
+            // The dot-product vector is s in the first loop and y in the
+            // second, the update vector the other one; a half of a lane's share
+            // past the vector's end reads zeros (kZeros) and is not computed.
+            double* recp[U];
+            double rv[U], av[U], dA[U][8], uB[U][8];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                if (u < take) {
-                    const int k = back ? pos + u - cnt : cnt - 1 - (pos + u);
-                    int slot = hd + k;
-                    if (slot >= cap) slot -= cap;
-                    const double* rec = ring + (long long)slot * R;
-                    rv[u] = rec[2 * N];
-                    av[u] = rec[2 * N + 1];
-                    if (live) {
+                int k = back ? pos + u - cnt : cnt - 1 - (pos + u);
+                k = (k < 0) ? 0 : k;
+                k = (k >= cnt) ? ((cnt > 0) ? cnt - 1 : 0) : k;
+                int slot = hd + k;
+                if (slot >= cap) slot -= cap;
+                double* const rec = ring + (long long)slot * R;
+                recp[u] = rec;
+                rv[u] = rec[2 * N];
+                av[u] = rec[2 * N + 1];
+                const double* const pa = rec + (back ? N : 0) + 8 * l;
+                const double* const pb = rec + (back ? 0 : N) + 8 * l;
+                const double2* const a_lo = reinterpret_cast<const double2*>(live ? pa : kZeros);
+                const double2* const a_hi = reinterpret_cast<const double2*>(upper ? pa + 4 : kZeros);
+                const double2* const b_lo = reinterpret_cast<const double2*>(live ? pb : kZeros);
+                const double2* const b_hi = reinterpret_cast<const double2*>(upper ? pb + 4 : kZeros);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            sv[u][c] = rec[8 * l + c];
-                            yv[u][c] = rec[N + 8 * l + c];
-                        }
-                    }
-                    if (upper) {
-#pragma unroll
-                        for (int c = 4; c < 8; ++c) {
-                            sv[u][c] = rec[8 * l + c];
-                            yv[u][c] = rec[N + 8 * l + c];
-                        }
-                    } else {
-#pragma unroll
-                        for (int c = 4; c < 8; ++c) {
-                            sv[u][c] = 0.0;
-                            yv[u][c] = 0.0;
-                        }
-                    }
+                for (int h = 0; h < 2; ++h) {
+                    const double2 x0 = a_lo[h], x1 = a_hi[h], z0 = b_lo[h], z1 = b_hi[h];
+                    dA[u][2 * h] = x0.x;
+                    dA[u][2 * h + 1] = x0.y;
+                    dA[u][4 + 2 * h] = x1.x;
+                    dA[u][4 + 2 * h + 1] = x1.y;
+                    uB[u][2 * h] = z0.x;
+                    uB[u][2 * h + 1] = z0.y;
+                    uB[u][4 + 2 * h] = z1.x;
+                    uB[u][4 + 2 * h + 1] = z1.y;
                 }
             }
             if (back && take > 0 && pos == cnt) {
@@ -747,35 +757,38 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (u < take) {
+                    // a half past the vector's end takes no part: its partial
+                    // sum is 0.0 and its components of q stay 0.0, as in the
+                    // oracle's tree order
                     double p0 = 0.0, p1 = 0.0;
                     if (live) {
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) p0 = p0 + (back ? yv[u][c] : sv[u][c]) * qv[c];
+                        for (int c = 0; c < 4; ++c) p0 = p0 + dA[u][c] * qv[c];
+                    }
+                    if (upper) {
 #pragma unroll
-                        for (int c = 4; c < 8; ++c) p1 = p1 + (back ? yv[u][c] : sv[u][c]) * qv[c];
+                        for (int c = 4; c < 8; ++c) p1 = p1 + dA[u][c] * qv[c];
                     }
                     const double dot = quad_sum(p0 + p1);
+                    const double sc = rv[u] * dot;  // a_k (first loop) or b (second)
+                    double coef;
                     if (!back) {
-                        const double a = rv[u] * dot;
-                        if (l == 0) {
-                            int slot = hd + cnt - 1 - (pos + u);
-                            if (slot >= cap) slot -= cap;
-                            ring[(long long)slot * R + 2 * N + 1] = a;
-                        }
-                        if (live) {
-#pragma unroll
-                            for (int c = 0; c < 8; ++c) qv[c] = qv[c] - a * yv[u][c];
-                        }
+                        if (l == 0) recp[u][2 * N + 1] = sc;
+                        coef = -sc;  // q - a y, as q + (-a) y: the same rounding
                     } else {
-                        const double b = rv[u] * dot;
-                        if (live) {
+                        coef = av[u] - sc;
+                    }
+                    if (live) {
 #pragma unroll
-                            for (int c = 0; c < 8; ++c) qv[c] = qv[c] + (av[u] - b) * sv[u][c];
-                        }
+                        for (int c = 0; c < 4; ++c) qv[c] = qv[c] + coef * uB[u][c];
+                    }
+                    if (upper) {
+#pragma unroll
+                        for (int c = 4; c < 8; ++c) qv[c] = qv[c] + coef * uB[u][c];
                     }
                 }
             }
-            pos += take;
+            pos += (take > 0) ? take : 0;
         }
         if (has) {
 #pragma unroll
