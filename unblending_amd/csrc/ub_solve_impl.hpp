@@ -1220,13 +1220,19 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             for (int j = 0; j < NC; ++j) lam[j] = lam[j] - rho * Ccur[j];
             const double ngn = norm<NC>(Ccur);
             if (ngn > sp.gamma * cn_outer) rho = rho * sp.beta;
+            // the previous outer iterate: every load issued before the first
+            // use, so that the lane waits for one memory round trip, not N
+            double xprev[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) xprev[i] = xo[i * 64];
             double dx2 = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const double dxv = xc[i] - xo[i * 64];
+                const double dxv = xc[i] - xprev[i];
                 dx2 += dxv * dxv;
-                xo[i * 64] = xc[i];
             }
+#pragma unroll
+            for (int i = 0; i < N; ++i) xo[i * 64] = xc[i];
             const bool unchanged = __builtin_sqrt(dx2) < sp.eps;
             const bool satisfied = ngn < sp.eps;
             cn_outer = ngn;
