@@ -13,7 +13,9 @@ solve starts.  What fixes each function here:
                 known values and the exact-erf form within 1e-3
   forward       a plain fp64 evaluation of the stated architecture
   loss_grad     fp64 backpropagation, pinned by central finite differences
-                of `loss` (a sign or index error in any layer fails them)
+                of `loss` (a sign or index error in any layer fails them);
+                its `plant` option computes the gradient with a named slip,
+                for tests of how far such slips move it
   adam          Kingma & Ba, Algorithm 1; the first step is -lr * g/(|g|+eps)
                 up to eps, a closed form
   splitmix64    Steele, Lea & Flood 2014; the first output of state 0 is the
@@ -135,9 +137,18 @@ def loss(params, X, T, outputs):
     return float(np.sum(e * e) / (X.shape[0] * outputs))
 
 
-def loss_grad(params, X, T, outputs, bf16_ops=False):
+# Derivative slips a backward pass could plausibly make, for tests that
+# check that the device gradient's tolerance would catch them (loss_grad's
+# `plant`): the GELU derivative without its second term, a hidden layer's
+# error propagated through W instead of W^T, the sigmoid's derivative left
+# out, the GELU derivative taken at the layer above's pre-activation.
+PLANTS = ("gelu_first_term", "untransposed_weight", "no_sigmoid_grad", "gelu_wrong_layer")
+
+
+def loss_grad(params, X, T, outputs, bf16_ops=False, plant=None):
     """Mean squared error over the valid outputs and its gradient with
-    respect to the padded parameter vector."""
+    respect to the padded parameter vector.  plant (one of PLANTS) computes
+    the gradient with that mistake, for tests of a tolerance's power."""
     W, _ = unpack(params)
     r = bf16 if bf16_ops else (lambda a: a)
     zs, hs, y = forward(params, X, bf16_ops)
@@ -146,14 +157,23 @@ def loss_grad(params, X, T, outputs, bf16_ops=False):
     mask[:outputs] = 1.0
     e = (y - T) * mask
     L = float(np.sum(e * e) / (B * outputs))
-    dz = 2.0 * e * y * (1.0 - y) / (B * outputs)
+    dsig = 1.0 if plant == "no_sigmoid_grad" else y * (1.0 - y)
+    dz = 2.0 * e * dsig / (B * outputs)
     gW, gb = [None] * 4, [None] * 4
     for l in range(3, -1, -1):
         dzr = r(dz)
         gW[l] = hs[l].T @ dzr
         gb[l] = dz.sum(axis=0)
         if l > 0:
-            dz = (dzr @ r(W[l]).T) * gelu_grad(zs[l - 1])
+            Wl = r(W[l])
+            back = dzr @ (Wl if (plant == "untransposed_weight" and Wl.shape[0] == Wl.shape[1]) else Wl.T)
+            if plant == "gelu_first_term":
+                t = np.tanh(GELU_C0 * (zs[l - 1] + GELU_C1 * zs[l - 1] ** 3))
+                dz = back * (0.5 * (1.0 + t))
+            elif plant == "gelu_wrong_layer" and zs[l].shape == zs[l - 1].shape:
+                dz = back * gelu_grad(zs[l])
+            else:
+                dz = back * gelu_grad(zs[l - 1])
     return L, pack(gW, gb)
 
 

@@ -4,7 +4,12 @@ The network runs BF16 matrix-core products with FP32 accumulation; its
 oracle (oracle/ub_mlp_oracle.py) rounds the same operands to BF16 and
 accumulates in FP64, so the bars are:
   forward    max |y_gpu - y_oracle| <= 2e-3 on the sigmoid outputs
-  gradient   per weight matrix / bias, |g_gpu - g_oracle| / |g_oracle| <= 2e-2
+  gradient   per weight matrix / bias, |g_gpu - g_oracle| / |g_oracle| <= 2e-3
+             (measured up to 1.8e-4: with the operands rounded alike, the
+             device's FP32 accumulation and the odd BF16 rounding flip are the
+             only differences; the derivative slips of
+             test_oracle_mlp.py::test_device_gradient_bar_catches_planted_slips
+             move some layer's gradient by 0.39 to 3.1)
   loss       relative 1e-3
   Adam       the FP32 update of the returned gradient within 1e-6 absolute
   sampling   the batch the device draws equals the oracle's SplitMix64 batch
@@ -24,7 +29,7 @@ from conftest import layer_infos, oracle_layers
 pytestmark = pytest.mark.gpu
 
 FORWARD_TOL = 2e-3
-GRAD_REL_TOL = 2e-2
+GRAD_REL_TOL = 2e-3
 LOSS_REL_TOL = 1e-3
 ADAM_TOL = 1e-6
 ALPHA_COLOR_TOL = 1e-3

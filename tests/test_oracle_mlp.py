@@ -150,3 +150,29 @@ def test_planted_mistake_fails_the_difference_check(monkeypatch):
         fd = (M.loss(p1, X, T, outputs) - M.loss(p2, X, T, outputs)) / 2e-6
         bad += abs(g[k] - fd) > 1e-5 * abs(fd) + 1e-11
     assert bad > 5
+
+
+@pytest.mark.parametrize("plant", M.PLANTS)
+def test_device_gradient_bar_catches_planted_slips(plant):
+    """The bar the device gradient is held to (test_gpu_mlp.GRAD_REL_TOL, per
+    weight matrix and bias, relative to the oracle) is passed by the device's
+    own rounding but not by a plausible derivative slip: each planted
+    mistake moves some layer's gradient, on the problem the GPU test uses,
+    by several times the bar."""
+    from test_gpu_mlp import GRAD_REL_TOL, _perturbed_params, _train_problem
+    L, B = 4, 256
+    images, targets = _train_problem(L)
+    count, h, w, _ = images.shape
+    params = _perturbed_params(L, 3)
+    idx = np.random.default_rng(4).integers(0, count * h * w, B)
+    X = M.patches(images, idx, w, h)
+    T = M.targets_of(targets, idx, L, w, h, 4 * L)
+    _, good = M.loss_grad(params, X, T, 4 * L, bf16_ops=True)
+    _, bad = M.loss_grad(params, X, T, 4 * L, bf16_ops=True, plant=plant)
+    worst = 0.0
+    for l, (i, o) in enumerate(M.SHAPES):
+        for off, size in ((M.OFF_W[l], i * o), (M.OFF_B[l], o)):
+            a, b = bad[off:off + size], good[off:off + size]
+            worst = max(worst, np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    print("%s: largest relative change %.3e" % (plant, worst))
+    assert worst > 5 * GRAD_REL_TOL
