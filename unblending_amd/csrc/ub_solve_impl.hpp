@@ -1161,13 +1161,15 @@ __device__ __forceinline__ void run_images(const BatchArgs& args, double* smem, 
             }
         }
         if constexpr (HIST) {
-            const unsigned long long pend = __ballot(phase == kPhDir);
-            unsigned long long done = 0;
+            // a direction with an empty history is q itself (gamma is 1 whenever
+            // the history is empty, and 1 q = q exactly): it needs no service
+            const unsigned long long pend = __ballot(phase == kPhDir && count > 0);
+            unsigned long long done = __ballot(phase == kPhDir && count == 0);
             if (pend) {
                 constexpr int R = kHistRecord(L);
                 const int cap = (&args.hist_cap)[zero];
                 const double* hblock = (&args.hist)[zero] + (long long)blockIdx.x * 64 * (long long)cap * R;
-                done = direction_service<L>(pend, smem + N * 64, hblock, cap, count, head, gamma, dstep);
+                done |= direction_service<L>(pend, smem + N * 64, hblock, cap, count, head, gamma, dstep);
             }
             if ((done >> lane) & 1ull) {
                 double qd[N];
