@@ -81,13 +81,35 @@ def _time_ms(fn, repeats):
     return best
 
 
-def filter_candidates(radius):
+def balanced_rows(width, height, radius, block, cus, rounds=(1, 2, 3, 4)):
+    """Rows per strip that make a one-wave-strip launch k whole rounds of
+    tiles on `cus` compute units (a pass-A tile is one workgroup of up to
+    eight waves, which a CU holds one of): a launch of a few tiles more than
+    a round leaves most CUs idle for its last round."""
+    out = []
+    tw = block - 2 * radius
+    if tw < 1 or cus < 1:
+        return out
+    strips = -(-width // tw)
+    for k in rounds:
+        chunks = max(1, (k * cus) // strips)
+        rows = max(32, -(-height // chunks))
+        if rows not in out:
+            out.append(rows)
+    return out
+
+
+def filter_candidates(radius, width=0, height=0, cus=0):
     out = []
     if 0 < radius <= FILTER_WAVE_MAX_RADIUS:
         for block in FILTER_WAVE_BLOCKS:
             if block - 2 * radius < 64:
                 continue
-            for rows in FILTER_ROWS:
+            rows_set = list(FILTER_ROWS)
+            for rows in balanced_rows(width, height, radius, block, cus):
+                if rows not in rows_set:
+                    rows_set.append(rows)
+            for rows in rows_set:
                 out.append((block, rows))
         return out
     for block in FILTER_BLOCKS:
@@ -105,7 +127,7 @@ def tune_filter(width, height, radius, channels, eps=1e-4, repeats=3, use_cache=
     width x height image with this radius (synthetic inputs of that shape)."""
     torch = _torch()
     from . import device
-    key = "%s|filter|%dx%d|r%d|c%d" % (_device_name(), width, height, radius, channels)
+    key = "%s|filter2|%dx%d|r%d|c%d" % (_device_name(), width, height, radius, channels)
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
@@ -117,7 +139,8 @@ def tune_filter(width, height, radius, channels, eps=1e-4, repeats=3, use_cache=
     n = (device._lib.unblending_hip_filter_workspace_size(width, height, channels) + 7) // 8
     ws = torch.empty(max(1, n), dtype=torch.float64, device="cuda")
     best = None
-    for block, rows in filter_candidates(radius):
+    cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    for block, rows in filter_candidates(radius, width, height, cus):
         ms = _time_ms(lambda: device.guided_filter(rgb, planes, radius, eps, out=out, ws=ws, block=block, rows=rows),
                       repeats)
         if log:
