@@ -668,7 +668,7 @@ struct PassA {
         v[2] = a.rgb[3 * pix + 2];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const Channel& c = a.in[a.first_channel + g];
+            const Channel& c = a.in[g];  // the launch's channels start at in[0]
             v[3 + g] = c.base[pix * c.stride];
         }
     }
@@ -734,7 +734,7 @@ struct PassB {
             q = q + qa[g].mean(H[4 * g + 0], weight) * I0;
             q = q + qa[g].mean(H[4 * g + 1], weight) * I1;
             q = q + qa[g].mean(H[4 * g + 2], weight) * I2;
-            a.q[a.first_channel + g][pix] = q;
+            a.q[g][pix] = q;  // the launch's channels start at q[0]
         }
     }
 };
@@ -960,9 +960,17 @@ hipError_t launch_pass(const FilterArgs& args, hipStream_t s)
     Geometry geo;
     if (!geometry(args.w, args.h, args.r, args.block, args.rows, geo)) return hipErrorInvalidValue;
     for (int g0 = 0; g0 < args.channels; g0 += kMaxGroup) {
+        // the launch's channels are moved to the front of in[] and q[], so that
+        // the kernels index those arrays with compile-time indices (a run-time
+        // index into the kernel argument would copy it to scratch memory);
+        // first_channel still places the group's bounds and coefficient planes
         FilterArgs a = args;
         a.first_channel = g0;
         a.groups = args.channels - g0 < kMaxGroup ? args.channels - g0 : kMaxGroup;
+        for (int c = 0; c < a.groups; ++c) {
+            a.in[c] = args.in[g0 + c];
+            a.q[c] = args.q[g0 + c];
+        }
         hipError_t e;
         switch (geo.E) {
         case 256: e = launch_e<PASS_A, 256>(a, geo, s); break;
