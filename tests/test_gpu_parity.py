@@ -116,6 +116,27 @@ def test_guided_filter_launch_shapes(oracle, block, rows):
         assert np.array_equal(q.cpu().numpy(), oracle.guided_filter_exact(p, rgb, r, eps))
 
 
+@pytest.mark.parametrize("w,h,r,block", [(130, 70, 100, 1024), (130, 70, 100, 512), (600, 50, 245, 0)])
+def test_guided_filter_wide_strips(oracle, w, h, r, block):
+    """Radii above 96 run workgroup strips of 256, 512 or 1024 threads; five
+    channels make launches of one channel (pass B at 1024 threads) and of
+    three and two (the other passes)."""
+    import torch
+    from unblending_amd import device
+    eps = 1e-4
+    rng = np.random.default_rng(w + h + r + block)
+    rgb = rng.uniform(0, 1, (h, w, 3))
+    planes = [rng.uniform(0, 1, (h, w)) * (0.4 + 0.15 * c) for c in range(5)]
+    print("case wide filter %dx%d r=%d block %d" % (w, h, r, block), flush=True)
+    out = device.guided_filter(torch.from_numpy(rgb).cuda(), [torch.from_numpy(p).cuda() for p in planes], r, eps,
+                               block=block)
+    torch.cuda.synchronize()
+    for p, q in zip(planes, out):
+        qn = q.cpu().numpy()
+        assert np.abs(qn - oracle.guided_filter(p, rgb, r, eps, brute=r <= 100)).max() <= FILTER_TOL
+        assert np.array_equal(qn, oracle.guided_filter_exact(p, rgb, r, eps))
+
+
 def test_refine_first_pass_may_alias_output():
     """The C ABI's refinement may take its first pass and its output in the
     same buffer (unblending/include/unblending.h:96-97): the result equals the

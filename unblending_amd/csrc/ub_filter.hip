@@ -959,14 +959,18 @@ hipError_t launch_pass(const FilterArgs& args, hipStream_t s)
     if (args.r <= kWaveStripMaxRadius) return launch_tiles<PASS_A>(args, s);
     Geometry geo;
     if (!geometry(args.w, args.h, args.r, args.block, args.rows, geo)) return hipErrorInvalidValue;
-    for (int g0 = 0; g0 < args.channels; g0 += kMaxGroup) {
+    // 1024-thread strips of pass B one channel a launch: with three their
+    // 12 running sums spill (128 registers a lane at that size), while pass A
+    // keeps three, since each launch recomputes the 9 guidance sums
+    const int gmax = (geo.E == 1024 && !PASS_A) ? 1 : kMaxGroup;
+    for (int g0 = 0; g0 < args.channels; g0 += gmax) {
         // the launch's channels are moved to the front of in[] and q[], so that
         // the kernels index those arrays with compile-time indices (a run-time
         // index into the kernel argument would copy it to scratch memory);
         // first_channel still places the group's bounds and coefficient planes
         FilterArgs a = args;
         a.first_channel = g0;
-        a.groups = args.channels - g0 < kMaxGroup ? args.channels - g0 : kMaxGroup;
+        a.groups = args.channels - g0 < gmax ? args.channels - g0 : gmax;
         for (int c = 0; c < a.groups; ++c) {
             a.in[c] = args.in[g0 + c];
             a.q[c] = args.q[g0 + c];
