@@ -645,9 +645,11 @@ constexpr int kDirBudget = 32;
 // the pixel's LDS column dv[i * 64 + lane] between trips.  Each call advances
 // every pending direction by at most kDirBudget steps, so that a pixel with a
 // long history does not hold up the evaluations of the rest of the wave.
-// The wave serves the pending pixels 16 at a time, longest remaining work
-// first: lane l of quad g (lanes 4g .. 4g + 3) holds components 8l .. 8l + 7
-// of one pending pixel and streams that pixel's records -- the k-th oldest at
+// The wave's 16 quads serve the pending pixels, longest remaining work first,
+// and a quad whose pixel is done takes the next one at once (so that a long
+// direction does not leave the other quads idle): lane l of quad g (lanes
+// 4g .. 4g + 3) holds components 8l .. 8l + 7 of the quad's pixel and
+// streams that pixel's records -- the k-th oldest at
 // slot (head + k) mod cap of its ring hist + p * cap * R -- kHistBlock
 // records a load batch.  4L is a multiple of 4, so each half (four
 // components) of a lane's share is wholly inside the vector or wholly past
@@ -664,143 +666,180 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
     constexpr int N = 4 * L;
     constexpr int R = kHistRecord(L);
     constexpr int U = kHistBlock;
-    const int lane = threadIdx.x, grp = lane >> 2, l = lane & 3;
+    const int lane = threadIdx.x, l = lane & 3;
     const bool live = 8 * l < N;       // components 8l .. 8l + 3 exist
     const bool upper = 8 * l + 8 <= N;  // components 8l + 4 .. 8l + 7 exist
     const bool mine = (pend >> lane) & 1ull;
-    // rank among the pending lanes: most remaining steps first
+    const int npend = __popcll(pend);
+    // rank among the pending lanes: most remaining steps first (the other
+    // lanes rank after them, in lane order, so that the ranks are a
+    // permutation of the lanes)
     const int rem = 2 * count - dstep;
-    int rank = 0;
+    int prank = 0;
     {
         unsigned long long m = pend;
         while (m) {
             const int o = __builtin_ctzll(m);
             m &= m - 1ull;
             const int ro = __builtin_amdgcn_readlane(rem, o);
-            rank += (ro > rem || (ro == rem && o < lane)) ? 1 : 0;
+            prank += (ro > rem || (ro == rem && o < lane)) ? 1 : 0;
         }
     }
+    const int rank = mine ? prank : npend + __popcll(~pend & ((1ull << lane) - 1ull));
+    // lane k of `order` holds the lane of rank k
+    const int order = __builtin_amdgcn_ds_permute(rank << 2, lane);
+    // lane k: the state of the rank-k pixel
+    const int o_cnt = __shfl(count, order);
+    const int o_hd = __shfl(head, order);
+    const int o_ds = __shfl(dstep, order);
+    const double o_gam = __shfl(gamma, order);
     // the records written at this trip's acceptances, by other lanes
     __threadfence_block();
-    const int rounds = (__popcll(pend) + 15) >> 4;
-    for (int r = 0; r < rounds; ++r) {
-        const unsigned long long rmask = __ballot(mine && (rank >> 4) == r);
-        unsigned long long m = rmask;
-        for (int t = 0; t < grp; ++t) m &= m - 1ull;
-        const bool has = m != 0ull;
-        const int p = has ? __builtin_ctzll(m) : 0;
-        const int pc = __shfl(count, p);
-        const int hd = __shfl(head, p);
-        const double gam = __shfl(gamma, p);
-        const int pd = __shfl(dstep, p);
-        const int cnt = has ? pc : 0;
-        int pos = has ? pd : 0;
-        const int end = has ? min(2 * cnt, pos + kDirBudget) : 0;
-        double* const ring = const_cast<double*>(hblock) + (long long)p * cap * R;
-        double qv[8];
+    // the pixel a quad serves (the lanes of a quad agree on all of these);
+    // a quad whose pixel is done for this call takes the next one by rank
+    bool has = false;
+    int p = 0, cnt = 0, hd = 0, pos = 0, end = 0;
+    double gam = 1.0;
+    double* ring = const_cast<double*>(hblock);
+    double qv[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) qv[c] = (has && 8 * l + c < N) ? dvcol[(8 * l + c) * 64 + p] : 0.0;
-        if (has && cnt == 0) {
+    for (int c = 0; c < 8; ++c) qv[c] = 0.0;
+    int next = 0;  // the next rank handed out (wave-uniform)
+    constexpr unsigned long long kQuadLead = 0x1111111111111111ull;
+    for (;;) {
+        const bool idle = !(pos < end);
+        unsigned long long fin = __ballot(idle && has);
+        if (fin) {
+            // the finished pixels' vectors back to their LDS columns, their
+            // step counts to their lanes
+            if (idle && has) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    if (8 * l + c < N) dvcol[(8 * l + c) * 64 + p] = qv[c];
+                }
+            }
+            fin &= kQuadLead;
+            while (fin) {
+                const int o = __builtin_ctzll(fin);
+                fin &= fin - 1ull;
+                const int po = __builtin_amdgcn_readlane(p, o);
+                const int so = __builtin_amdgcn_readlane(pos, o);
+                if (lane == po) dstep = so;
+            }
+            if (idle) has = false;
+        }
+        const unsigned long long qm = __ballot(idle) & kQuadLead;
+        if (next < npend && qm) {
+            const int nr = next + __popcll(qm & ((1ull << (lane & ~3)) - 1ull));
+            const int nl = nr < 64 ? nr : 0;
+            const int src = __shfl(order, nl);
+            const int pc = __shfl(o_cnt, nl);
+            const int ph = __shfl(o_hd, nl);
+            const int pd = __shfl(o_ds, nl);
+            const double pg = __shfl(o_gam, nl);
+            next += __popcll(qm);
+            if (idle && nr < npend) {
+                has = true;
+                p = src;
+                cnt = pc;
+                hd = ph;
+                gam = pg;
+                pos = pd;
+                end = min(2 * cnt, pos + kDirBudget);
+                ring = const_cast<double*>(hblock) + (long long)p * cap * R;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) qv[c] = (8 * l + c < N) ? dvcol[(8 * l + c) * 64 + p] : 0.0;
+                if (cnt == 0) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
+                }
+            }
+        }
+        if (!__ballot(pos < end)) break;
+        // a batch stays within one loop of the recursion (its loads of a_k
+        // follow the stores of the first loop in the wave's program order,
+        // which the wave's memory operations to one address keep)
+        const int lim = (pos < cnt && cnt < end) ? cnt : end;
+        const int take = min(U, lim - pos);  // <= 0 for a quad whose work is done
+        const bool back = pos >= cnt;        // second loop, oldest to newest
+        // Every lane loads U records a batch: record indices are clamped
+        // to the ring (a quad with fewer records left re-reads one), so the
+        // loads are unconditional and their registers carry nothing from
+        // one batch to the next; records past `take` are not computed.
+        // The dot-product vector is s in the first loop and y in the
+        // second, the update vector the other one; a half of a lane's share
+        // past the vector's end reads zeros (kZeros) and is not computed.
+        double* recp[U];
+        double rv[U], av[U], dA[U][8], uB[U][8];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int k = back ? pos + u - cnt : cnt - 1 - (pos + u);
+            k = (k < 0) ? 0 : k;
+            k = (k >= cnt) ? ((cnt > 0) ? cnt - 1 : 0) : k;
+            int slot = hd + k;
+            if (slot >= cap) slot -= cap;
+            double* const rec = ring + (long long)slot * R;
+            recp[u] = rec;
+            rv[u] = rec[2 * N];
+            av[u] = rec[2 * N + 1];
+            const double* const pa = rec + (back ? N : 0) + 8 * l;
+            const double* const pb = rec + (back ? 0 : N) + 8 * l;
+            const double2* const a_lo = reinterpret_cast<const double2*>(live ? pa : kZeros);
+            const double2* const a_hi = reinterpret_cast<const double2*>(upper ? pa + 4 : kZeros);
+            const double2* const b_lo = reinterpret_cast<const double2*>(live ? pb : kZeros);
+            const double2* const b_hi = reinterpret_cast<const double2*>(upper ? pb + 4 : kZeros);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double2 x0 = a_lo[h], x1 = a_hi[h], z0 = b_lo[h], z1 = b_hi[h];
+                dA[u][2 * h] = x0.x;
+                dA[u][2 * h + 1] = x0.y;
+                dA[u][4 + 2 * h] = x1.x;
+                dA[u][4 + 2 * h + 1] = x1.y;
+                uB[u][2 * h] = z0.x;
+                uB[u][2 * h + 1] = z0.y;
+                uB[u][4 + 2 * h] = z1.x;
+                uB[u][4 + 2 * h + 1] = z1.y;
+            }
+        }
+        if (back && take > 0 && pos == cnt) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
         }
-        while (__ballot(pos < end)) {
-            // a batch stays within one loop of the recursion (its loads of a_k
-            // follow the stores of the first loop in the wave's program order,
-            // which the wave's memory operations to one address keep)
-            const int lim = (pos < cnt && cnt < end) ? cnt : end;
-            const int take = min(U, lim - pos);  // <= 0 for a quad whose work is done
-            const bool back = pos >= cnt;        // second loop, oldest to newest
-            // Every lane loads U records a batch: record indices are clamped
-            // to the ring (a quad with fewer records left re-reads one), so the
-            // loads are unconditional and their registers carry nothing from
-            // one batch to the next; records past `take` are not computed.
-            // The dot-product vector is s in the first loop and y in the
-            // second, the update vector the other one; a half of a lane's share
-            // past the vector's end reads zeros (kZeros) and is not computed.
-            double* recp[U];
-            double rv[U], av[U], dA[U][8], uB[U][8];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                int k = back ? pos + u - cnt : cnt - 1 - (pos + u);
-                k = (k < 0) ? 0 : k;
-                k = (k >= cnt) ? ((cnt > 0) ? cnt - 1 : 0) : k;
-                int slot = hd + k;
-                if (slot >= cap) slot -= cap;
-                double* const rec = ring + (long long)slot * R;
-                recp[u] = rec;
-                rv[u] = rec[2 * N];
-                av[u] = rec[2 * N + 1];
-                const double* const pa = rec + (back ? N : 0) + 8 * l;
-                const double* const pb = rec + (back ? 0 : N) + 8 * l;
-                const double2* const a_lo = reinterpret_cast<const double2*>(live ? pa : kZeros);
-                const double2* const a_hi = reinterpret_cast<const double2*>(upper ? pa + 4 : kZeros);
-                const double2* const b_lo = reinterpret_cast<const double2*>(live ? pb : kZeros);
-                const double2* const b_hi = reinterpret_cast<const double2*>(upper ? pb + 4 : kZeros);
+        for (int u = 0; u < U; ++u) {
+            if (u < take) {
+                // a half past the vector's end takes no part: its partial
+                // sum is 0.0 and its components of q stay 0.0, as in the
+                // oracle's tree order
+                double p0 = 0.0, p1 = 0.0;
+                if (live) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double2 x0 = a_lo[h], x1 = a_hi[h], z0 = b_lo[h], z1 = b_hi[h];
-                    dA[u][2 * h] = x0.x;
-                    dA[u][2 * h + 1] = x0.y;
-                    dA[u][4 + 2 * h] = x1.x;
-                    dA[u][4 + 2 * h + 1] = x1.y;
-                    uB[u][2 * h] = z0.x;
-                    uB[u][2 * h + 1] = z0.y;
-                    uB[u][4 + 2 * h] = z1.x;
-                    uB[u][4 + 2 * h + 1] = z1.y;
+                    for (int c = 0; c < 4; ++c) p0 = p0 + dA[u][c] * qv[c];
+                }
+                if (upper) {
+#pragma unroll
+                    for (int c = 4; c < 8; ++c) p1 = p1 + dA[u][c] * qv[c];
+                }
+                const double dot = quad_sum(p0 + p1);
+                const double sc = rv[u] * dot;  // a_k (first loop) or b (second)
+                double coef;
+                if (!back) {
+                    if (l == 0) recp[u][2 * N + 1] = sc;
+                    coef = -sc;  // q - a y, as q + (-a) y: the same rounding
+                } else {
+                    coef = av[u] - sc;
+                }
+                if (live) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) qv[c] = qv[c] + coef * uB[u][c];
+                }
+                if (upper) {
+#pragma unroll
+                    for (int c = 4; c < 8; ++c) qv[c] = qv[c] + coef * uB[u][c];
                 }
             }
-            if (back && take > 0 && pos == cnt) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (u < take) {
-                    // a half past the vector's end takes no part: its partial
-                    // sum is 0.0 and its components of q stay 0.0, as in the
-                    // oracle's tree order
-                    double p0 = 0.0, p1 = 0.0;
-                    if (live) {
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) p0 = p0 + dA[u][c] * qv[c];
-                    }
-                    if (upper) {
-#pragma unroll
-                        for (int c = 4; c < 8; ++c) p1 = p1 + dA[u][c] * qv[c];
-                    }
-                    const double dot = quad_sum(p0 + p1);
-                    const double sc = rv[u] * dot;  // a_k (first loop) or b (second)
-                    double coef;
-                    if (!back) {
-                        if (l == 0) recp[u][2 * N + 1] = sc;
-                        coef = -sc;  // q - a y, as q + (-a) y: the same rounding
-                    } else {
-                        coef = av[u] - sc;
-                    }
-                    if (live) {
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) qv[c] = qv[c] + coef * uB[u][c];
-                    }
-                    if (upper) {
-#pragma unroll
-                        for (int c = 4; c < 8; ++c) qv[c] = qv[c] + coef * uB[u][c];
-                    }
-                }
-            }
-            pos += (take > 0) ? take : 0;
         }
-        if (has) {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                if (8 * l + c < N) dvcol[(8 * l + c) * 64 + p] = qv[c];
-            }
-        }
-        // the owner of each served pixel takes its new step count from the
-        // quad's first lane
-        const int own_g = __popcll(rmask & ((1ull << lane) - 1ull));
-        const int np = __shfl(pos, 4 * (own_g & 15));
-        if (mine && (rank >> 4) == r) dstep = np;
+        pos += (take > 0) ? take : 0;
     }
     return __ballot(mine && dstep == 2 * count);
 }
