@@ -632,9 +632,10 @@ __device__ __forceinline__ double quad_sum(double v)
 __device__ double kZeros[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
 // Records of one history ring streamed per load batch of the two-loop
-// recursion, and the recursion steps a pending direction advances per trip.
-constexpr int kHistBlock = 3;
-constexpr int kDirBudget = 32;
+// recursion (three at L = 8, whose refinement kernel already spills), and the
+// recursion steps a pending direction advances per trip.
+constexpr int kHistBlock(int L) { return L >= 8 ? 3 : 4; }
+constexpr int kDirBudget = 48;
 
 // Search directions from the (s, y) history for the lanes in `pend` (those
 // whose direction is pending).  A direction of a pixel with `count` records
@@ -665,7 +666,7 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
 {
     constexpr int N = 4 * L;
     constexpr int R = kHistRecord(L);
-    constexpr int U = kHistBlock;
+    constexpr int U = kHistBlock(L);
     const int lane = threadIdx.x, l = lane & 3;
     const bool live = 8 * l < N;       // components 8l .. 8l + 3 exist
     const bool upper = 8 * l + 8 <= N;  // components 8l + 4 .. 8l + 7 exist
