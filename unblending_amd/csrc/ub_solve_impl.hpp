@@ -654,7 +654,8 @@ constexpr int kDirBudget = 48;
 // slot (head + k) mod cap of its ring hist + p * cap * R -- kHistBlock
 // records a load batch.  4L is a multiple of 4, so each half (four
 // components) of a lane's share is wholly inside the vector or wholly past
-// its end; a half past the end is not loaded and holds zeros.  A dot product
+// its end; a half past the end reads zeros (kZeros) and takes no part in the
+// arithmetic.  A dot product
 // is each lane's two four-component partial sums, added, then summed over
 // the quad (quad_sum).  A quad's trip counts are uniform over the quad, so
 // its DPP moves run on whole quads.  Returns the lanes whose direction is
