@@ -147,7 +147,7 @@ def cpu_baseline(wl, rows, threads, bounds=None, extra_configs=True):
     dt1, _, _ = oracle_pass_pair(wl, srows, 1, bounds)
     rec = {"value": n / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
            "target_concurrency": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
-           "filter": "exact-sum guided filter (the device's definition; within 1e-9 of the FP64 brute-force one)",
+           "filter": "exact-sum guided filter (the device's definition; within 1e-9 of the FP64 brute-force one at r <= 96)",
            "inner_minimiser": "L-BFGS with NLopt's default history (every pair of an inner solve)",
            "sample": "oracle (FP64 C restatement), unmix + refine of rows 0-%d of the same image "
                      "(%d x %d px), %d threads, %.1f s" % (rows - 1, wl.width, rows, threads, dt),

@@ -248,6 +248,22 @@ def test_exact_filter_is_the_guided_filter_to_rounding(oracle, w, h, r, eps):
     assert np.abs(qe - qb).max() < 1e-9
 
 
+@pytest.mark.parametrize("w,h,r", [(3, 200, 100), (200, 3, 100), (70, 5, 130), (2, 2, 480), (5, 9, 300)])
+def test_exact_filter_at_large_radii(oracle, w, h, r):
+    """Beyond r = 96 the partial sums run over 1024 columns of wider windows,
+    so the grids are coarser.  The coarsest is the coefficients' (|a_c| <=
+    max|p| / sqrt(eps) = 100 here: a step of 2^-26 at r = 480, half of it
+    the rounding error of each of a_r, a_g, a_b, and b's twice that), so
+    the exact filter stays within about 4e-8 of the FP64 brute-force one at
+    eps = 1e-4 (measured up to 2.2e-8), below the 1e-6 bar."""
+    rng = np.random.default_rng(w * 1000 + h + r)
+    I = rng.uniform(0, 1, (h, w, 3))
+    p = rng.uniform(0, 1, (h, w))
+    qe = oracle.guided_filter_exact(p, I, r, 1e-4)
+    qb = oracle.guided_filter(p, I, r, 1e-4, brute=True)
+    assert np.abs(qe - qb).max() < 5e-8
+
+
 @pytest.mark.parametrize("eps", [1e-4, 1e-3, 1e-2])
 def test_coefficient_bound_holds_with_margin(eps):
     """The grids of a and b assume |a|_2 <= sd(p) / (2 sqrt(eps)) (the
