@@ -333,3 +333,24 @@ def test_planted_jacobian_mistake_is_caught(oracle):
     bad[3 + 3:3 + 6, :3] = bad[3 + 3:3 + 6, :3].T.copy()
     bad[1, :3], bad[1, 3] = J[1, 3], J[1, 0]
     assert np.abs(bad - fd).max() > 1e-3
+
+
+@pytest.mark.parametrize("k,size", [(1, (64, 32)), (3, (48, 24))])
+def test_warm_start_at_the_plain_solution_moves(oracle, k, size):
+    """A warm start cannot reproduce the default start's outputs to 1e-3: even
+    started at the default start's own solution (multipliers at zero and rho
+    at rho0, as any start has them), the solve ends more than 1e-3 away for
+    most pixels -- the first outer subproblems move it, and the 5e-3
+    tolerances (unblending.cpp:221-222) stop it at another accepted point."""
+    from unblending_amd import synth
+    wl = synth.config(k, size)
+    ol = oracle_layers(wl.layers)
+    rgb = wl.rgb.reshape(-1, 3)
+    idx = np.random.default_rng(0).choice(len(rgb), 200, replace=False)
+    moved = []
+    for i in idx:
+        x = np.asarray(oracle.solve_pixel(rgb[i], ol, opaque=True)[0])
+        w = np.asarray(oracle.solve_pixel_warm(rgb[i], ol, x, opaque=True)[0])
+        moved.append(np.abs(w - x).max())
+    moved = np.array(moved)
+    assert (moved > 1e-3).mean() > 0.5
