@@ -652,7 +652,9 @@ constexpr int kDirBudget = 48;
 // 4g .. 4g + 3) holds components 8l .. 8l + 7 of the quad's pixel and
 // streams that pixel's records -- the k-th oldest at
 // slot (head + k) mod cap of its ring hist + p * cap * R -- kHistBlock
-// records a load batch.  4L is a multiple of 4, so each half (four
+// records a load batch; the batch that ends the first loop also runs the
+// second loop's first steps on the records it holds (the oldest), their
+// multipliers kept in registers.  4L is a multiple of 4, so each half (four
 // components) of a lane's share is wholly inside the vector or wholly past
 // its end; a half past the end reads zeros (kZeros) and takes no part in the
 // arithmetic.  A dot product
@@ -807,8 +809,14 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
 #pragma unroll
             for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
         }
+        // the batch that ends the first loop (its records are the oldest
+        // `take`) also runs the second loop's first `take` steps on them, the
+        // multipliers kept in registers, when the trip's budget has room
+        const bool turn = !back && take > 0 && pos + take == cnt && pos + 2 * take <= end;
+        double ak[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+            ak[u] = 0.0;
             if (u < take) {
                 // a half past the vector's end takes no part: its partial
                 // sum is 0.0 and its components of q stay 0.0, as in the
@@ -826,7 +834,8 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
                 const double sc = rv[u] * dot;  // a_k (first loop) or b (second)
                 double coef;
                 if (!back) {
-                    if (l == 0) recp[u][2 * N + 1] = sc;
+                    if (l == 0 && !turn) recp[u][2 * N + 1] = sc;
+                    ak[u] = sc;
                     coef = -sc;  // q - a y, as q + (-a) y: the same rounding
                 } else {
                     coef = av[u] - sc;
@@ -841,7 +850,36 @@ __device__ __forceinline__ unsigned long long direction_service(unsigned long lo
                 }
             }
         }
-        pos += (take > 0) ? take : 0;
+        if (turn) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) qv[c] = gam * qv[c];
+            // oldest first: record k = take - 1 - u sits in slot u
+#pragma unroll
+            for (int u = U - 1; u >= 0; --u) {
+                if (u < take) {
+                    double p0 = 0.0, p1 = 0.0;
+                    if (live) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) p0 = p0 + uB[u][c] * qv[c];
+                    }
+                    if (upper) {
+#pragma unroll
+                        for (int c = 4; c < 8; ++c) p1 = p1 + uB[u][c] * qv[c];
+                    }
+                    const double dot = quad_sum(p0 + p1);
+                    const double coef = ak[u] - rv[u] * dot;
+                    if (live) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) qv[c] = qv[c] + coef * dA[u][c];
+                    }
+                    if (upper) {
+#pragma unroll
+                        for (int c = 4; c < 8; ++c) qv[c] = qv[c] + coef * dA[u][c];
+                    }
+                }
+            }
+        }
+        pos += (take > 0) ? (turn ? 2 * take : take) : 0;
     }
     return __ballot(mine && dstep == 2 * count);
 }
