@@ -71,6 +71,20 @@ def mlp_train_flops(batch: int) -> float:
     return 2.0 * batch * (2 * fwd + 32 * 128 + 2 * 128 * 128)
 
 
+def profiled_filter_traffic(w, h, L):
+    """HBM bytes of one filter (bounds pass, pass A, pass B) at this workload,
+    from the committed copy-calibrated rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, "profiles", "solve_history_c3_4096_hbm_traffic.json")
+    if (w, h, L) != (4096, 4096, 7) or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        prof = json.load(f)["kernels"]
+    keys = ("filter_bounds", "filter_pass_a", "filter_pass_b")
+    if not all(k in prof for k in keys):
+        return None, None
+    return sum(prof[k]["bytes"] for k in keys), "profiles/" + os.path.basename(path) + " (default launch shape)"
+
+
 def profiled_traffic(w, h, L, refine, memory):
     """HBM bytes per dispatch of the solve kernel, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or
@@ -107,6 +121,38 @@ def first_pass_quality(packed, rgb, layers):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def stream_rates(torch, mib: int, reps: int = 10) -> dict:
+    """HBM rates of plain streaming kernels over `mib` MiB of float64 (torch's
+    own kernels, HIP events, best of `reps`): a device-to-device copy (read +
+    write), a full read (a sum) and a fill (write only).  They bound what any
+    HBM-bound kernel here can reach."""
+    n = mib << 17
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    acc = torch.empty((), dtype=torch.float64, device="cuda")
+
+    def best(fn, nbytes):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        times = []
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1) * 1e-3)
+        return nbytes / min(times) / 1e9
+
+    rec = {"mib": mib, "reps": reps,
+           "copy_gbs": best(lambda: dst.copy_(src), 16 * n),
+           "read_gbs": best(lambda: torch.sum(src, dim=0, out=acc), 8 * n),
+           "fill_gbs": best(lambda: dst.fill_(1.0), 8 * n),
+           "peak_gbs": 8000.0}
+    for k in ("copy", "read", "fill"):
+        rec[k + "_frac"] = rec[k + "_gbs"] / rec["peak_gbs"]
+    return rec
 
 
 def cpu_model() -> str:
@@ -157,12 +203,32 @@ def cpu_baseline(wl, rows, threads, bounds=None, extra_configs=True):
         whole = {}
         for k in (1, 2):
             wk = synth.config(k)
-            dk, _, _ = oracle_pass_pair(wk, None, threads)
+            dk, fk, _ = oracle_pass_pair(wk, None, threads)
             whole["c%d" % k] = {"workload": wk.name, "value": wk.width * wk.height / dk / 1e6, "unit": "Mpixel/s",
                                 "seconds": round(dk, 2), "threads": threads}
             log("oracle %s whole: %.1f s" % (wk.name, dk))
+            if k == 1:
+                whole["c1_filter"] = oracle_filter_times(wk, fk)
         rec["whole_configs"] = whole
     return rec, first, second
+
+
+def oracle_filter_times(wl, first):
+    """The oracle's three guided filters on a workload's first-pass alpha
+    planes, one thread: the reference's clamp-edge (2r + 1)^2 box
+    (image_processing.cpp:120-151), the sliding-window FP64 box sums, and the
+    device's exact-sum definition; seconds for all L planes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ub_oracle as O
+    rec = {"workload": wl.name, "planes": len(wl.layers), "radius": wl.radius, "eps": wl.eps, "threads": 1}
+    for name, fn in (("brute_force_s", lambda p: O.guided_filter(p, wl.rgb, wl.radius, wl.eps, brute=True)),
+                     ("sliding_s", lambda p: O.guided_filter(p, wl.rgb, wl.radius, wl.eps, brute=False)),
+                     ("exact_s", lambda p: O.guided_filter_exact(p, wl.rgb, wl.radius, wl.eps))):
+        t0 = time.perf_counter()
+        for l in range(len(wl.layers)):
+            fn(np.ascontiguousarray(first[l, ..., 3]))
+        rec[name] = round(time.perf_counter() - t0, 3)
+    return rec
 
 
 def run_batch_bench(args):
@@ -355,6 +421,7 @@ def main():
     ap.add_argument("--lbfgs-memory", type=int, default=0,
                     help="(s, y) pairs of the inner minimiser: 0 = NLopt's default (every pair), 1 = fast mode")
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-abi", action="store_true", help="skip timing the step through the host-pointer C interface")
     ap.add_argument("--no-fast", action="store_true", help="skip the one-pair fast-mode comparison")
     ap.add_argument("--no-cpu-configs", action="store_true", help="skip timing the oracle on c1 and c2 whole")
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -365,6 +432,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the reference point "
                          "for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--stream-rates", type=int, default=0, metavar="MIB",
+                    help="only time plain streaming kernels over MIB MiB (device-to-device copy, a full read, "
+                         "a fill) and exit: the rates an HBM-bound kernel can reach on this device")
     ap.add_argument("--c5-images", type=int, default=64)
     ap.add_argument("--c5-scale", type=float, default=1.0)
     ap.add_argument("--c5-train-steps", type=int, default=1000, help="training steps of each per-set network")
@@ -376,6 +446,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.stream_rates:
+        print(json.dumps(stream_rates(torch, args.stream_rates)), flush=True)
+        return
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration << 17, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -468,6 +541,25 @@ def main():
     recon_plain_first, rc_first = recon(first_plain)
     recon_plain_refined, rc_refined = recon(out_plain)
     side_steps, side_warm = max(1, min(args.steps, 5)), min(args.warmup, 1)
+
+    # ---- the same step through the host-pointer C interface, once ----
+    abi = None
+    if not args.no_abi:
+        rgb_host = np.ascontiguousarray(wl.rgb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a_first = U.compute_color_unmixing(rgb_host, infos, True, params=params)
+        t1 = time.perf_counter()
+        a_out = U.perform_matte_refinement(rgb_host, a_first, infos, True, False, params=params)
+        t2 = time.perf_counter()
+        abi = {"value": round(n / (t2 - t0) / 1e6, 4), "unit": "Mpixel/s", "unmix_s": round(t1 - t0, 3),
+               "refine_s": round(t2 - t1, 3),
+               "bit_identical_to_device_path": bool(np.array_equal(a_first, first_plain.cpu().numpy()) and
+                                                    np.array_equal(a_out, out_plain.cpu().numpy())),
+               "note": "unblending_unmix_ex + unblending_refine_ex on host buffers, one call each: device "
+                       "allocation, host-device copies and repacks included; not the value"}
+        log("host-pointer interface: %.2f s + %.2f s" % (t1 - t0, t2 - t1))
+        del a_first, a_out
 
     # ---- the one-pair fast mode, against the reported step's outputs ----
     fast = None
@@ -571,6 +663,10 @@ def main():
     tflops = dflops / (ks.solve_ms * 1e-3) / 1e12
     traffic, traffic_source = profiled_traffic(w, h, L, refine_dom, args.lbfgs_memory)
     value = n / (ms_plain * 1e-3) / 1e6
+    # what plain streaming kernels reach on this device, measured after the
+    # timed steps: the practical ceiling for the HBM-bound kernels above
+    stream = stream_rates(torch, 1024, reps=5)
+    ftraffic, ftraffic_source = profiled_filter_traffic(w, h, L)
 
     rec = {
         "metric": "Mpixel/s unmix+refine, plain vs MLP warm-start at equal error; filter HBM GB/s",
@@ -595,13 +691,18 @@ def main():
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_source,
                      "bytes_per_launch": dbytes, "launch_ms": round(ks.solve_ms, 3),
+                     "vs_copy_rate": round(gbs / stream["copy_gbs"], 4),
                      "fp64": {"achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flops_per_launch": dflops}},
         "filter": {"achieved": round(fgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(fgbs / HBM_PEAK_GBS, 4), "ms": round(filter_ms, 3),
                    "coeff_ms": round(s2.filter_coeff_ms, 3), "out_ms": round(s2.filter_out_ms, 3),
                    "bytes": fb, "channels": s2.filter_channels,
-                   "timed": "bounds pass + pass A + pass B"},
+                   "timed": "bounds pass + pass A + pass B",
+                   "vs_copy_rate": round(fgbs / stream["copy_gbs"], 4),
+                   "compulsory_bytes": 8.0 * (3 + 2 * s2.filter_channels) * n,
+                   "traffic": ftraffic, "traffic_source": ftraffic_source},
+        "hbm_stream": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in stream.items()},
         "plain": {"unmix_ms": round(s1.solve_ms, 3), "refine_ms": round(s2.solve_ms, 3),
                   "unmix_evals_per_px": s1.evals / n, "unmix_iters_per_px": s1.iterations / n,
                   "unmix_outer_per_px": s1.outer / n, "refine_evals_per_px": s2.evals / n,
@@ -619,6 +720,8 @@ def main():
                                          / (s2.solve_ms * 1e-3) / 1e12, 3),
                   "recomposite": {"first": recon_plain_first, "refined": recon_plain_refined}},
     }
+    if abi:
+        rec["host_interface"] = abi
     if fast:
         rec["fast_mode"] = fast
     if warm:
