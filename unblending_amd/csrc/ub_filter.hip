@@ -284,6 +284,9 @@ __device__ __forceinline__ double lane_fetch(double v, int byte_index)
 // One-wave strips (r <= 96)
 // ===========================================================================
 
+// Two consecutive doubles at an 8-byte-aligned address: one 16-byte access.
+typedef double dpair __attribute__((ext_vector_type(2), aligned(8)));
+
 // K values of a channel from the columns cols[] (clamped) of a row
 template <int K>
 __device__ __forceinline__ void load_channel(const Channel& ch, long long rowbase, const int* cols, double* v, int step)
@@ -371,6 +374,17 @@ struct ChanSrc {
 #pragma unroll
         for (int k = 0; k < 4; ++k) ab[(long long)k * n + pix] = c[k];
     }
+    // the coefficients of K consecutive output pixels pix0 ..., 16-byte stores
+    template <int K>
+    __device__ __forceinline__ void out_run(long long pix0, const double c[][4]) const
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double* p = ab + (long long)k * n + pix0;
+#pragma unroll
+            for (int j = 0; j < K; j += 2) *(dpair*)(p + j) = dpair{c[j][k], c[j + 1][k]};
+        }
+    }
 };
 
 // Pass B of one channel: raw inputs and statistics (a_r, a_g, a_b, b),
@@ -423,6 +437,30 @@ struct WaveB {
         v = v + qa.mean(H[1], weight) * I1;
         v = v + qa.mean(H[2], weight) * I2;
         q[pix] = v;
+    }
+    // K consecutive output pixels pix0 ...: 16-byte guidance loads and stores
+    template <int K>
+    __device__ __forceinline__ void out_run(long long pix0, const double H[][NS]) const
+    {
+        double I[3 * K];
+        const double* g = rgb + 3 * pix0;
+#pragma unroll
+        for (int i = 0; i < 3 * K; i += 2) {
+            const dpair d = *(const dpair*)(g + i);
+            I[i] = d.x;
+            I[i + 1] = d.y;
+        }
+        double v[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            double t = qb.mean(H[j][3], weight);
+            t = t + qa.mean(H[j][0], weight) * I[3 * j];
+            t = t + qa.mean(H[j][1], weight) * I[3 * j + 1];
+            t = t + qa.mean(H[j][2], weight) * I[3 * j + 2];
+            v[j] = t;
+        }
+#pragma unroll
+        for (int j = 0; j < K; j += 2) *(dpair*)(q + pix0 + j) = dpair{v[j], v[j + 1]};
     }
 };
 
@@ -591,15 +629,30 @@ __global__ void __launch_bounds__(512) coeff_tile_kernel(FilterArgs a, int TW, i
         double H[K][4];
         st.window(H);
         __syncthreads();
+        const int c0 = lane * K;  // the lane's first strip column; column c's window is strip columns c - 2r .. c
+        const int x0 = X0 + c0 - 2 * r;
+        if (c0 >= 2 * r && c0 + K - 1 - 2 * r < TW && x0 + K - 1 < a.w) {
+            // all K columns are output columns
+            double cf[K][4];
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const int c = lane * K + j;  // strip column; its window is strip columns c - 2r .. c
-            const int x = X0 + c - 2 * r;
-            if (c >= 2 * r && c - 2 * r < TW && x < a.w) {
+            for (int j = 0; j < K; ++j) {
                 double gi[9];
 #pragma unroll
                 for (int k = 0; k < 9; ++k) gi[k] = gt[buf][k][j][lane];
-                src.out((long long)y * a.w + x, gi, H[j]);
+                channel_coefficients(gi, H[j], src.qc, src.weight, cf[j]);
+            }
+            src.template out_run<K>((long long)y * a.w + x0, cf);
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const int c = c0 + j;
+                const int x = x0 + j;
+                if (c >= 2 * r && c - 2 * r < TW && x < a.w) {
+                    double gi[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) gi[k] = gt[buf][k][j][lane];
+                    src.out((long long)y * a.w + x, gi, H[j]);
+                }
             }
         }
         buf ^= 1;
@@ -632,11 +685,17 @@ __global__ void __launch_bounds__(512) output_tile_kernel(FilterArgs a, int TW, 
         st.advance(src, y, y == y0, y + 1 < y1);
         double H[K][4];
         st.window(H);
+        const int c0 = lane * K;
+        const int x0 = X0 + c0 - 2 * r;
+        if (c0 >= 2 * r && c0 + K - 1 - 2 * r < TW && x0 + K - 1 < a.w) {
+            src.template out_run<K>((long long)y * a.w + x0, H);
+        } else {
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const int c = lane * K + j;
-            const int x = X0 + c - 2 * r;
-            if (c >= 2 * r && c - 2 * r < TW && x < a.w) src.out((long long)y * a.w + x, H[j]);
+            for (int j = 0; j < K; ++j) {
+                const int c = c0 + j;
+                const int x = x0 + j;
+                if (c >= 2 * r && c - 2 * r < TW && x < a.w) src.out((long long)y * a.w + x, H[j]);
+            }
         }
     }
 }
