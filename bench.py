@@ -82,7 +82,7 @@ def profiled_filter_traffic(w, h, L):
     keys = ("filter_bounds", "filter_pass_a", "filter_pass_b")
     if not all(k in prof for k in keys):
         return None, None
-    return sum(prof[k]["bytes"] for k in keys), "profiles/" + os.path.basename(path) + " (default launch shape)"
+    return sum(prof[k]["bytes"] for k in keys), "profiles/" + os.path.basename(path)
 
 
 def profiled_traffic(w, h, L, refine, memory):
