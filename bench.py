@@ -189,8 +189,10 @@ def cpu_baseline(wl, rows, threads, bounds=None, extra_configs=True):
     from unblending_amd import synth
     dt, first, second = oracle_pass_pair(wl, rows, threads, bounds)
     n = rows * wl.width
-    srows = max(1, rows // 16)
+    log("oracle band of %d rows on %d threads: %.1f s" % (rows, threads, dt))
+    srows = max(1, min(rows // 16, 64))
     dt1, _, _ = oracle_pass_pair(wl, srows, 1, bounds)
+    log("oracle band of %d rows on 1 thread: %.1f s" % (srows, dt1))
     rec = {"value": n / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
            "target_concurrency": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
            "filter": "exact-sum guided filter (the device's definition; within 1e-9 of the FP64 brute-force one at r <= 96)",
@@ -731,7 +733,8 @@ def main():
         bounds = np.array([float(rgb.abs().max())] + [float(first_plain[l, ..., 3].abs().max()) for l in range(L)])
         cpu, ofirst, osecond = cpu_baseline(wl, args.cpu_rows, threads, bounds, extra_configs=not args.no_cpu_configs)
         rows = args.cpu_rows
-        keep = max(0, rows - 2 * wl.radius - 2)  # refinement rows unaffected by the band's lower edge
+        # refinement rows unaffected by the band's lower edge (all of them for the whole image)
+        keep = max(0, rows - 2 * wl.radius - 2) if rows < h else h
         g1 = first_plain[:, :rows].cpu().numpy()
         g2 = out_plain[:, :keep].cpu().numpy()
         cpu["plain_vs_oracle"] = {"first_max": float(np.abs(g1 - ofirst).max()),
