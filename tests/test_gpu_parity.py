@@ -116,6 +116,27 @@ def test_guided_filter_launch_shapes(oracle, block, rows):
         assert np.array_equal(q.cpu().numpy(), oracle.guided_filter_exact(p, rgb, r, eps))
 
 
+@pytest.mark.parametrize("w,h,r,block", [(389, 41, 32, 256), (390, 37, 17, 256), (391, 35, 30, 128),
+                                         (645, 19, 31, 256), (388, 33, 5, 128)])
+def test_guided_filter_strip_edges(oracle, w, h, r, block):
+    """Several strips a row, widths of every residue mod 4 and radii of both
+    parities: lanes whose K columns are all output columns store them as
+    16-byte pairs, the others (at each strip's halo and the image's right
+    edge) column by column; both must give the exact filter."""
+    import torch
+    from unblending_amd import device
+    eps = 1e-3
+    rng = np.random.default_rng(w + 7 * h + 31 * r + block)
+    rgb = rng.uniform(0, 1, (h, w, 3))
+    planes = [rng.uniform(0, 1, (h, w)) * (0.6 + 0.1 * c) for c in range(3)]
+    print("case filter strip edges %dx%d r=%d block %d" % (w, h, r, block), flush=True)
+    out = device.guided_filter(torch.from_numpy(rgb).cuda(), [torch.from_numpy(p).cuda() for p in planes], r, eps,
+                               block=block)
+    torch.cuda.synchronize()
+    for p, q in zip(planes, out):
+        assert np.array_equal(q.cpu().numpy(), oracle.guided_filter_exact(p, rgb, r, eps))
+
+
 @pytest.mark.parametrize("w,h,r,block", [(130, 70, 100, 1024), (130, 70, 100, 512), (600, 50, 245, 0)])
 def test_guided_filter_wide_strips(oracle, w, h, r, block):
     """Radii above 96 run workgroup strips of 256, 512 or 1024 threads; five
