@@ -8,12 +8,16 @@
 // its prologue).
 // The host-pointer path (host_solve) adds the uploads and the download, as
 // the reference's solve_cuda does (pixel_pipeline.cu:63-105), on a stream of
-// its own, and returns after synchronising it.
+// its own, and returns after synchronising it; the packed planes travel
+// through pinned staging buffers.
 #include <algorithm>
 #include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 #include "ub_host.hpp"
 
@@ -115,6 +119,138 @@ struct Stream {
     Stream() { check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
     ~Stream() { if (s) (void)hipStreamDestroy(s); }
 };
+
+// memcpy of a large block by up to 8 host threads (parts of at least 4 MiB);
+// with fewer threads available the calling thread copies the rest.
+void parallel_copy(void* dst, const void* src, size_t bytes)
+{
+    constexpr size_t kMinPart = size_t(4) << 20;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(hw ? hw : 1, 8));
+    const size_t parts = std::min(threads, (bytes + kMinPart - 1) / kMinPart);
+    auto copy_part = [dst, src](size_t off, size_t len) {
+        if (len) std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, len);
+    };
+    if (parts <= 1) {
+        copy_part(0, bytes);
+        return;
+    }
+    const size_t step = (bytes + parts - 1) / parts;
+    auto part_len = [&](size_t p) { return p * step < bytes ? std::min(step, bytes - p * step) : size_t(0); };
+    std::vector<std::thread> pool;
+    pool.reserve(parts - 1);
+    size_t started = 1;  // part 0 is the calling thread's
+    try {
+        for (; started < parts; ++started) pool.emplace_back(copy_part, started * step, part_len(started));
+    } catch (...) {
+        // no more threads: the parts not started are copied below
+    }
+    copy_part(0, part_len(0));
+    for (size_t p = started; p < parts; ++p) copy_part(p * step, part_len(p));
+    for (auto& t : pool) t.join();
+}
+
+// Copies between the caller's (pageable) host buffers and the device through
+// two pinned staging buffers: host threads fill or drain one while the DMA
+// engine moves the other.  The runtime's own copies from pageable memory
+// run at a fraction of the link's rate (measured on an MI355X host: 30 GB/s
+// up and 18 GB/s down, 57 GB/s from pinned memory), and a download into a
+// freshly allocated result also pays its page faults on one thread.
+class Staging {
+public:
+    static constexpr size_t kChunk = size_t(64) << 20;
+
+    Staging()
+    {
+        for (int b = 0; b < 2; ++b) {
+            if (hipHostMalloc(&buf_[b], kChunk, hipHostMallocDefault) != hipSuccess) buf_[b] = nullptr;
+            if (hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming) != hipSuccess) ev_[b] = nullptr;
+        }
+    }
+    ~Staging()
+    {
+        for (int b = 0; b < 2; ++b) {
+            if (ev_[b]) (void)hipEventDestroy(ev_[b]);
+            if (buf_[b]) (void)hipHostFree(buf_[b]);
+        }
+    }
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    bool ok() const { return buf_[0] && buf_[1] && ev_[0] && ev_[1]; }
+
+    // bytes from host src to device dst on stream s; returns when they are
+    // on the device (work queued on s before the call runs first)
+    void upload(void* dst, const void* src, size_t bytes, hipStream_t s)
+    {
+        bool used[2] = {false, false};
+        for (size_t i = 0, off = 0; off < bytes; ++i, off += kChunk) {
+            const int b = int(i & 1);
+            const size_t len = std::min(kChunk, bytes - off);
+            if (used[b]) check(hipEventSynchronize(ev_[b]), "upload");  // its previous chunk has left
+            parallel_copy(buf_[b], static_cast<const char*>(src) + off, len);
+            check(hipMemcpyAsync(static_cast<char*>(dst) + off, buf_[b], len, hipMemcpyHostToDevice, s), "upload");
+            check(hipEventRecord(ev_[b], s), "upload");
+            used[b] = true;
+        }
+        // the last chunks' copies must finish before their buffers are reused
+        for (int b = 0; b < 2; ++b)
+            if (used[b]) check(hipEventSynchronize(ev_[b]), "upload");
+    }
+
+    // bytes from device src (ready once the work queued on s is done) to
+    // host dst; returns when all of them are in dst
+    void download(void* dst, const void* src, size_t bytes, hipStream_t s)
+    {
+        const size_t chunks = (bytes + kChunk - 1) / kChunk;
+        auto len_of = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
+        auto issue = [&](size_t i) {
+            const int b = int(i & 1);
+            check(hipMemcpyAsync(buf_[b], static_cast<const char*>(src) + i * kChunk, len_of(i),
+                                 hipMemcpyDeviceToHost, s),
+                  "download");
+            check(hipEventRecord(ev_[b], s), "download");
+        };
+        if (chunks) issue(0);
+        for (size_t i = 0; i < chunks; ++i) {
+            // chunk i + 1 goes to the buffer whose chunk i - 1 was drained below
+            if (i + 1 < chunks) issue(i + 1);
+            const int b = int(i & 1);
+            check(hipEventSynchronize(ev_[b]), "download");
+            parallel_copy(static_cast<char*>(dst) + i * kChunk, buf_[b], len_of(i));
+        }
+    }
+
+private:
+    void* buf_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
+// The staging buffers of the host-pointer path, kept for the life of the
+// process (one set; a call that finds them in use by another host thread
+// stages through a set of its own).
+std::mutex g_staging_mutex;
+Staging* g_staging = nullptr;  // never freed: it lives until the process ends
+
+// Runs fn(staging) with the shared staging set, or a private one if another
+// thread holds it, or with nullptr when pinned memory cannot be had.
+template <class Fn>
+void with_staging(Fn fn)
+{
+    std::unique_lock<std::mutex> lock(g_staging_mutex, std::try_to_lock);
+    if (lock.owns_lock()) {
+        if (!g_staging) {
+            std::unique_ptr<Staging> st(new Staging());
+            if (st->ok()) g_staging = st.release();
+        }
+        if (g_staging) {
+            fn(g_staging);
+            return;
+        }
+        lock.unlock();
+    }
+    Staging own;
+    fn(own.ok() ? &own : nullptr);
+}
 
 void require_device()
 {
@@ -658,9 +794,14 @@ void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const doubl
     check(hipMemcpy2DAsync(d_rgb.p, 3 * w * sizeof(double), rgb.data, rgb.row_stride * sizeof(double),
                            3 * w * sizeof(double), h, hipMemcpyHostToDevice, stream.s),
           "upload rgb");
-    if (first)
-        check(hipMemcpyAsync(d_first.p, first, 4 * L * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
-              "upload first pass");
+    if (first) {
+        with_staging([&](Staging* st) {
+            if (st) st->upload(d_first.p, first, 4 * L * n * sizeof(double), stream.s);
+            else
+                check(hipMemcpyAsync(d_first.p, first, 4 * L * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
+                      "upload first pass");
+        });
+    }
     DeviceJob job;
     job.w = int(w);
     job.h = int(h);
@@ -677,9 +818,15 @@ void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const doubl
     job.stats = stats;
     run_device(job);
     check(hipStreamSynchronize(stream.s), "solve");
-    check(hipMemcpyAsync(output, d_out.p, 4 * L * n * sizeof(double), hipMemcpyDeviceToHost, stream.s),
-          "download");
-    check(hipStreamSynchronize(stream.s), "download");
+    with_staging([&](Staging* st) {
+        if (st) {
+            st->download(output, d_out.p, 4 * L * n * sizeof(double), stream.s);
+        } else {
+            check(hipMemcpyAsync(output, d_out.p, 4 * L * n * sizeof(double), hipMemcpyDeviceToHost, stream.s),
+                  "download");
+            check(hipStreamSynchronize(stream.s), "download");
+        }
+    });
 }
 
 // Image-level recomposite (unblending_hip_composite*): comp_ops [count][3]
