@@ -74,6 +74,27 @@ def test_planar_seam_matches_oracle(oracle, k, size, smooth):
     assert np.array_equal(packed, second)
 
 
+@pytest.mark.gpu
+def test_planar_seam_threaded_repack_matches_packed():
+    """At 1024 x 640 the seam's repacks run on several host threads (ranges
+    of at least 2^18 pixels); both passes through the planes equal the packed
+    C interface bit for bit."""
+    import unblending_amd as U
+    from unblending_amd import synth
+    wl = synth.config(1, (1024, 640))
+    L = len(wl.layers)
+    h, w, _ = wl.rgb.shape
+    infos = layer_infos(wl.layers)
+    print("case planes threaded repack %dx%d" % (w, h), flush=True)
+    st, p1 = _solve_planes(wl, None, True, False)
+    assert st == U.OK, U.lib().unblending_hip_last_error()
+    first = from_planes(p1, L, h, w)
+    assert np.array_equal(first, U.compute_color_unmixing(wl.rgb, infos, True))
+    st, p2 = _solve_planes(wl, to_planes(first), True, False)
+    assert st == U.OK, U.lib().unblending_hip_last_error()
+    assert np.array_equal(from_planes(p2, L, h, w), U.perform_matte_refinement(wl.rgb, first, infos, True, False))
+
+
 def test_planar_seam_rejects_bad_arguments_before_allocating():
     import unblending_amd as U
     from unblending_amd import synth

@@ -10,9 +10,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ub_host.hpp"
@@ -24,6 +26,37 @@ thread_local std::string g_last_error;
 namespace {
 
 enum class Backend { automatic, cpu, cuda, hip };
+
+// f(begin, end) over [0, n) in up to 8 ranges of at least 2^18 items on host
+// threads; with fewer threads available the calling thread runs the rest.
+template <class F>
+void parallel_for(size_t n, F f)
+{
+    constexpr size_t kMinRange = size_t(1) << 18;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(hw ? hw : 1, 8));
+    const size_t parts = std::min(threads, (n + kMinRange - 1) / kMinRange);
+    if (parts <= 1) {
+        f(size_t(0), n);
+        return;
+    }
+    const size_t step = (n + parts - 1) / parts;
+    auto range = [&](size_t p) {
+        const size_t b = std::min(n, p * step);
+        f(b, std::min(n, b + step));
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(parts - 1);
+    size_t started = 1;  // range 0 is the calling thread's
+    try {
+        for (; started < parts; ++started) pool.emplace_back(range, started);
+    } catch (...) {
+        // no more threads: the ranges not started run below
+    }
+    range(0);
+    for (size_t p = started; p < parts; ++p) range(p);
+    for (auto& t : pool) t.join();
+}
 
 // unblending_c.cpp:12-21, plus the HIP value.
 Backend select_backend(int32_t backend)
@@ -588,33 +621,41 @@ UNBLENDING_API int32_t uhip_solve_planes(int32_t width, int32_t height, const un
         const size_t n = size_t(width) * size_t(height), L = layer_count;
         if (n > std::vector<double>().max_size() / 4 / L) return UNBLENDING_INVALID_ARGUMENT;
         // gpu_dispatch.cpp:136-158 planes -> packed, and back (:182-191).
-        std::vector<double> rgb(3 * n), first, out(4 * L * n);
-        for (size_t i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) rgb[3 * i + c] = rgb_planes[c * n + i];
+        // the intermediate packed buffers are written in full before they are
+        // read: allocated without zero-filling (3.8 GB each at 4096^2, L = 7)
+        std::unique_ptr<double[]> rgb(new double[3 * n]), first, out(new double[4 * L * n]);
+        parallel_for(n, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i)
+                for (int c = 0; c < 3; ++c) rgb[3 * i + c] = rgb_planes[c * n + i];
+        });
         if (first_planes) {
-            first.resize(4 * L * n);
-            for (size_t l = 0; l < L; ++l)
-                for (size_t i = 0; i < n; ++i) {
-                    double* px = first.data() + 4 * (l * n + i);
-                    for (int c = 0; c < 3; ++c) px[c] = first_planes[(L + 3 * l + c) * n + i];
-                    px[3] = first_planes[l * n + i];
-                }
+            first.reset(new double[4 * L * n]);
+            parallel_for(n, [&](size_t b, size_t e) {
+                for (size_t l = 0; l < L; ++l)
+                    for (size_t i = b; i < e; ++i) {
+                        double* px = first.get() + 4 * (l * n + i);
+                        for (int c = 0; c < 3; ++c) px[c] = first_planes[(L + 3 * l + c) * n + i];
+                        px[3] = first_planes[l * n + i];
+                    }
+            });
         }
-        unblending_rgb_view view{rgb.data(), rgb.size(), width, height, 0};
+        unblending_rgb_view view{rgb.get(), 3 * n, width, height, 0};
         unblending_options opt = UNBLENDING_OPTIONS_INIT;
         opt.backend = UNBLENDING_BACKEND_HIP;
         opt.opaque_background = opaque ? 1 : 0;
         opt.smooth_background = smooth ? 1 : 0;
         const int32_t status = first_planes
-            ? unblending_refine(&view, layers, L, first.data(), first.size(), &opt, out.data(), out.size())
-            : unblending_unmix(&view, layers, L, &opt, out.data(), out.size());
+            ? unblending_refine(&view, layers, L, first.get(), 4 * L * n, &opt, out.get(), 4 * L * n)
+            : unblending_unmix(&view, layers, L, &opt, out.get(), 4 * L * n);
         if (status != UNBLENDING_OK) return status;
-        for (size_t l = 0; l < L; ++l)
-            for (size_t i = 0; i < n; ++i) {
-                const double* px = out.data() + 4 * (l * n + i);
-                for (int c = 0; c < 3; ++c) out_planes[(L + 3 * l + c) * n + i] = px[c];
-                out_planes[l * n + i] = px[3];
-            }
+        parallel_for(n, [&](size_t b, size_t e) {
+            for (size_t l = 0; l < L; ++l)
+                for (size_t i = b; i < e; ++i) {
+                    const double* px = out.get() + 4 * (l * n + i);
+                    for (int c = 0; c < 3; ++c) out_planes[(L + 3 * l + c) * n + i] = px[c];
+                    out_planes[l * n + i] = px[3];
+                }
+        });
         return UNBLENDING_OK;
     } catch (const std::bad_alloc&) {
         g_last_error = "out of memory";
