@@ -856,11 +856,21 @@ void host_composite(const double* layers, int w, int h, size_t count, const int3
     Stream stream;
     DeviceBuffer d_in(4 * count * n * sizeof(double));
     DeviceBuffer d_out(4 * n * sizeof(double));
-    check(hipMemcpyAsync(d_in.p, layers, 4 * count * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
-          "upload layers");
+    with_staging([&](Staging* st) {
+        if (st) st->upload(d_in.p, layers, 4 * count * n * sizeof(double), stream.s);
+        else
+            check(hipMemcpyAsync(d_in.p, layers, 4 * count * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
+                  "upload layers");
+    });
     run_composite(d_in.as<double>(), w, h, count, comp_ops, modes, d_out.as<double>(), stream.s);
-    check(hipMemcpyAsync(out, d_out.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream.s), "download");
-    check(hipStreamSynchronize(stream.s), "composite");
+    with_staging([&](Staging* st) {
+        if (st) {
+            st->download(out, d_out.p, 4 * n * sizeof(double), stream.s);
+        } else {
+            check(hipMemcpyAsync(out, d_out.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream.s), "download");
+            check(hipStreamSynchronize(stream.s), "composite");
+        }
+    });
 }
 
 }  // namespace ubhip
