@@ -185,21 +185,26 @@ def test_refine_first_pass_may_alias_output():
 
 
 def test_concurrent_calls_from_two_threads():
-    """The C ABI is re-entrant (each call has its own stream and buffers): two
-    host threads unmixing different images at once get the results of the
-    same calls made one after the other."""
+    """The C ABI is re-entrant (each call has its own stream and buffers, and
+    a call that finds the shared pinned staging buffers in use stages through
+    its own): two host threads unmixing and refining different images at
+    once get the results of the same calls made one after the other."""
     import threading
     import unblending_amd as U
     work = [_workload(1, (40, 36)), _workload(2, (28, 44))]
     print("case two threads c1 40x36 + c2 28x44", flush=True)
     alone = [U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True) for wl in work]
+    alone2 = [U.perform_matte_refinement(wl.rgb, a, layer_infos(wl.layers), True, False)
+              for wl, a in zip(work, alone)]
     got = [None, None]
+    got2 = [None, None]
     errors = []
 
     def run(i):
         try:
             for _ in range(3):
                 got[i] = U.compute_color_unmixing(work[i].rgb, layer_infos(work[i].layers), True)
+                got2[i] = U.perform_matte_refinement(work[i].rgb, got[i], layer_infos(work[i].layers), True, False)
         except Exception as e:  # reported below, after the join
             errors.append(e)
 
@@ -212,3 +217,4 @@ def test_concurrent_calls_from_two_threads():
     assert not errors, errors
     for i in range(2):
         assert np.array_equal(got[i], alone[i])
+        assert np.array_equal(got2[i], alone2[i])
