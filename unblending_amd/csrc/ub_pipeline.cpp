@@ -155,52 +155,52 @@ void parallel_copy(void* dst, const void* src, size_t bytes)
 // engine moves the other.  The runtime's own copies from pageable memory
 // run at a fraction of the link's rate (measured on an MI355X host: 30 GB/s
 // up and 18 GB/s down, 57 GB/s from pinned memory), and a download into a
-// freshly allocated result also pays its page faults on one thread.
+// freshly allocated result also pays its page faults on one thread.  The
+// pinned buffers serve any device; the two events that order their reuse
+// are created for each copy on the current device.
 class Staging {
 public:
     static constexpr size_t kChunk = size_t(64) << 20;
 
     Staging()
     {
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < 2; ++b)
             if (hipHostMalloc(&buf_[b], kChunk, hipHostMallocDefault) != hipSuccess) buf_[b] = nullptr;
-            if (hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming) != hipSuccess) ev_[b] = nullptr;
-        }
     }
     ~Staging()
     {
-        for (int b = 0; b < 2; ++b) {
-            if (ev_[b]) (void)hipEventDestroy(ev_[b]);
+        for (int b = 0; b < 2; ++b)
             if (buf_[b]) (void)hipHostFree(buf_[b]);
-        }
     }
     Staging(const Staging&) = delete;
     Staging& operator=(const Staging&) = delete;
-    bool ok() const { return buf_[0] && buf_[1] && ev_[0] && ev_[1]; }
+    bool ok() const { return buf_[0] && buf_[1]; }
 
     // bytes from host src to device dst on stream s; returns when they are
     // on the device (work queued on s before the call runs first)
     void upload(void* dst, const void* src, size_t bytes, hipStream_t s)
     {
+        Events2 ev;
         bool used[2] = {false, false};
         for (size_t i = 0, off = 0; off < bytes; ++i, off += kChunk) {
             const int b = int(i & 1);
             const size_t len = std::min(kChunk, bytes - off);
-            if (used[b]) check(hipEventSynchronize(ev_[b]), "upload");  // its previous chunk has left
+            if (used[b]) check(hipEventSynchronize(ev.e[b]), "upload");  // its previous chunk has left
             parallel_copy(buf_[b], static_cast<const char*>(src) + off, len);
             check(hipMemcpyAsync(static_cast<char*>(dst) + off, buf_[b], len, hipMemcpyHostToDevice, s), "upload");
-            check(hipEventRecord(ev_[b], s), "upload");
+            check(hipEventRecord(ev.e[b], s), "upload");
             used[b] = true;
         }
         // the last chunks' copies must finish before their buffers are reused
         for (int b = 0; b < 2; ++b)
-            if (used[b]) check(hipEventSynchronize(ev_[b]), "upload");
+            if (used[b]) check(hipEventSynchronize(ev.e[b]), "upload");
     }
 
     // bytes from device src (ready once the work queued on s is done) to
     // host dst; returns when all of them are in dst
     void download(void* dst, const void* src, size_t bytes, hipStream_t s)
     {
+        Events2 ev;
         const size_t chunks = (bytes + kChunk - 1) / kChunk;
         auto len_of = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
         auto issue = [&](size_t i) {
@@ -208,31 +208,49 @@ public:
             check(hipMemcpyAsync(buf_[b], static_cast<const char*>(src) + i * kChunk, len_of(i),
                                  hipMemcpyDeviceToHost, s),
                   "download");
-            check(hipEventRecord(ev_[b], s), "download");
+            check(hipEventRecord(ev.e[b], s), "download");
         };
         if (chunks) issue(0);
         for (size_t i = 0; i < chunks; ++i) {
             // chunk i + 1 goes to the buffer whose chunk i - 1 was drained below
             if (i + 1 < chunks) issue(i + 1);
             const int b = int(i & 1);
-            check(hipEventSynchronize(ev_[b]), "download");
+            check(hipEventSynchronize(ev.e[b]), "download");
             parallel_copy(static_cast<char*>(dst) + i * kChunk, buf_[b], len_of(i));
         }
     }
 
 private:
+    struct Events2 {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        Events2()
+        {
+            for (int b = 0; b < 2; ++b) {
+                const hipError_t r = hipEventCreateWithFlags(&e[b], hipEventDisableTiming);
+                if (r != hipSuccess) {
+                    e[b] = nullptr;
+                    if (b) (void)hipEventDestroy(e[0]);
+                    check(r, "hipEventCreate");
+                }
+            }
+        }
+        ~Events2()
+        {
+            for (int b = 0; b < 2; ++b)
+                if (e[b]) (void)hipEventDestroy(e[b]);
+        }
+    };
     void* buf_[2] = {nullptr, nullptr};
-    hipEvent_t ev_[2] = {nullptr, nullptr};
 };
 
 // The staging buffers of the host-pointer path, kept for the life of the
-// process (one set; a call that finds them in use by another host thread
-// stages through a set of its own).
+// process (one pair; a call that finds them in use by another host thread
+// stages through a pair of its own).
 std::mutex g_staging_mutex;
 Staging* g_staging = nullptr;  // never freed: it lives until the process ends
 
-// Runs fn(staging) with the shared staging set, or a private one if another
-// thread holds it, or with nullptr when pinned memory cannot be had.
+// Runs fn(staging) with the shared staging buffers, or a private pair if
+// another thread holds them, or with nullptr when pinned memory cannot be had.
 template <class Fn>
 void with_staging(Fn fn)
 {
