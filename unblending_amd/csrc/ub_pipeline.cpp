@@ -595,9 +595,16 @@ namespace {
 // q_offset[i] doubles into the planes area).
 struct BatchLayout {
     size_t desc, lane_state, ab, planes, hist, total;
+    size_t ab_stride = 0;  // one coefficient region per concurrent filter stream
+    int filter_streams = 1;
     std::vector<size_t> q_offset;
     int hist_blocks = 0, hist_cap = 0;
 };
+
+// Images of a batch filtered at once, each stream with its own coefficient
+// planes: a batch of small images (c5: 64 images of 0.03-4 Mpixel) fills only
+// part of the device with one image's filter launches.
+constexpr int kFilterStreams = 4;
 
 BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool refine, bool smooth,
                          const unblending_hip_params& params)
@@ -617,7 +624,9 @@ BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool 
             planes += (size_t(C) * n + 31) / 32 * 32;
         }
     }
-    b.planes = b.ab + align256(ab_bytes);
+    b.filter_streams = refine ? int(std::min<size_t>(kFilterStreams, std::max<size_t>(count, 1))) : 1;
+    b.ab_stride = align256(ab_bytes);
+    b.planes = b.ab + size_t(b.filter_streams) * b.ab_stride;
     b.hist = b.planes + align256(planes * sizeof(double));
     // one history region, shared by the launches of the batch (one after
     // another on the stream), sized for the largest layer count's records
@@ -636,6 +645,48 @@ BatchLayout batch_layout(const unblending_hip_image* images, size_t count, bool 
 }
 
 }  // namespace
+
+// The streams a batch's filters run on besides the caller's, and the events
+// that fork them from and join them to the caller's stream: one set per
+// device, created at the first batch refinement on it and kept until the
+// process ends (never created during a graph capture: a batch captured before
+// its device has them filters on the caller's stream).  The mutex keeps one
+// batch at a time between a fork and its join.
+constexpr int kMaxDevices = 64;
+struct FilterSide {
+    std::mutex mutex;
+    hipStream_t s[kFilterStreams] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kFilterStreams] = {};
+};
+
+FilterSide* filter_side(hipStream_t caller)
+{
+    static std::mutex sides_mutex;
+    static FilterSide* sides[kMaxDevices] = {};  // never freed
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> guard(sides_mutex);
+    if (sides[dev]) return sides[dev];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(caller, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+    std::unique_ptr<FilterSide> f(new FilterSide());
+    bool ok = hipEventCreateWithFlags(&f->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < kFilterStreams; ++k)
+        ok = hipStreamCreateWithFlags(&f->s[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&f->join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        for (int k = 0; k < kFilterStreams; ++k) {
+            if (f->s[k]) (void)hipStreamDestroy(f->s[k]);
+            if (f->join[k]) (void)hipEventDestroy(f->join[k]);
+        }
+        if (f->fork) (void)hipEventDestroy(f->fork);
+        return nullptr;
+    }
+    sides[dev] = f.release();
+    return sides[dev];
+}
 
 size_t batch_workspace_bytes(const unblending_hip_image* images, size_t count, bool refine, bool smooth,
                              const unblending_hip_params& params)
@@ -675,15 +726,35 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
         if (stats) stats->predict_ms = ep.ms();
     }
 
-    // refinement: filter every image's alphas (and background) into its planes
+    // refinement: filter every image's alphas (and background) into its
+    // planes, up to kFilterStreams images at once: each image goes to the
+    // stream with the fewest pixels so far (largest images first), with that
+    // stream's coefficient region
     if (refine) {
-        double* ab = reinterpret_cast<double*>(ws + lay.ab);
+        FilterSide* side = lay.filter_streams > 1 ? filter_side(stream) : nullptr;
+        std::unique_lock<std::mutex> side_lock;
+        if (side) side_lock = std::unique_lock<std::mutex>(side->mutex);
+        const int lanes = side ? lay.filter_streams : 1;
+        std::vector<size_t> by_size(count);
+        for (size_t i = 0; i < count; ++i) by_size[i] = i;
+        auto pixels = [&](size_t i) { return size_t(images[i].width) * size_t(images[i].height); };
+        std::stable_sort(by_size.begin(), by_size.end(), [&](size_t a, size_t b) { return pixels(a) > pixels(b); });
+        std::vector<size_t> load(lanes, 0);
+        Events ew(timing);
+        ew.start(stream);
+        if (side) {
+            check(hipEventRecord(side->fork, stream), "filter fork");
+            for (int k = 0; k < lanes; ++k) check(hipStreamWaitEvent(side->s[k], side->fork, 0), "filter fork");
+        }
         unblending_hip_stats fs{};
-        double filter_ms = 0.0, coeff_ms = 0.0, out_ms = 0.0;
         int launches = 0;
-        for (size_t i = 0; i < count; ++i) {
+        for (size_t i : by_size) {
+            const int k = int(std::min_element(load.begin(), load.end()) - load.begin());
+            load[k] += pixels(i);
+            hipStream_t fstream = side ? side->s[k] : stream;
+            double* ab = reinterpret_cast<double*>(ws + lay.ab + size_t(k) * lay.ab_stride);
             const unblending_hip_image& im = images[i];
-            const size_t n = size_t(im.width) * size_t(im.height), L = sets[i].host.size();
+            const size_t n = pixels(i), L = sets[i].host.size();
             int radius = im.filter_radius;
             if (radius < 0) radius = 60 * std::min(im.width, im.height) / 1000;  // unblending.cpp:351
             const double eps = params.filter_epsilon > 0.0 ? params.filter_epsilon : 1e-4;  // :353
@@ -694,17 +765,23 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
                 in[l] = im.alpha_planes ? Channel{im.alpha_planes + l * n, 1} : Channel{im.first_pass + 4 * l * n + 3, 4};
             for (int c = 0; c < (smooth ? 3 : 0); ++c) in[L + c] = Channel{im.first_pass + c, 4};
             for (int c = 0; c < C; ++c) q[c] = planes + lay.q_offset[i] + size_t(c) * n;
-            run_filter(im.rgb, im.width, im.height, in.data(), q.data(), C, radius, eps, ab, stream,
-                       timing ? &fs : nullptr, timing, params.filter_block, params.filter_rows);
-            filter_ms += fs.filter_ms;
-            coeff_ms += fs.filter_coeff_ms;
-            out_ms += fs.filter_out_ms;
+            // untimed per image (a per-image timing would wait for each
+            // filter in turn): the phase is timed as a whole below
+            run_filter(im.rgb, im.width, im.height, in.data(), q.data(), C, radius, eps, ab, fstream,
+                       stats ? &fs : nullptr, false, params.filter_block, params.filter_rows);
             launches += fs.filter_launches;
         }
+        if (side) {
+            for (int k = 0; k < lanes; ++k) {
+                check(hipEventRecord(side->join[k], side->s[k]), "filter join");
+                check(hipStreamWaitEvent(stream, side->join[k], 0), "filter join");
+            }
+        }
+        ew.stop(stream);
         if (stats) {
-            stats->filter_ms = filter_ms;
-            stats->filter_coeff_ms = coeff_ms;
-            stats->filter_out_ms = out_ms;
+            stats->filter_ms = timing ? ew.ms() : 0.0;
+            stats->filter_coeff_ms = 0.0;  // the passes of concurrent images are not timed apart
+            stats->filter_out_ms = 0.0;
             stats->filter_launches = launches;
         }
     }
