@@ -87,6 +87,26 @@ def test_batch_refine_matches_single_images():
         assert torch.equal(single, im.out), wl.name
 
 
+def test_batch_refine_many_images_over_the_filter_streams():
+    """Nine images (more than the four filter streams, so that streams take
+    several images in turn, each into its stream's coefficient region), with
+    the smoothed background's three extra channels: every image equals its
+    single-image refinement."""
+    import torch
+    import unblending_amd as U
+    from unblending_amd import device
+    wls = _images(9, 0.05, seed0=60)
+    print("case batch refine over streams: " + ", ".join("%s r=%d" % (wl.name, wl.radius) for wl in wls), flush=True)
+    first = _batch(wls)
+    device.unmix_batch(first)
+    imgs = _batch(wls, firsts=[f.out for f in first])
+    device.refine_batch(imgs, smooth=True)
+    torch.cuda.synchronize()
+    for wl, f, im in zip(wls, first, imgs):
+        single = device.refine(im.rgb, f.out, im.layer_infos, True, True, params=U.hip_params(radius=wl.radius))
+        assert torch.equal(single, im.out), wl.name
+
+
 def test_batch_matches_oracle(oracle):
     from unblending_amd import device
     wls = _images(3, 0.03, seed0=30)
