@@ -237,28 +237,46 @@ struct EpiPartial {
     }
 };
 
-// out[m][n] = sum over z, in order, of partial[z][m][n]
-__global__ void split_sum_kernel(const float* partial, int splits, long long M, int N, float* out)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M * N) return;
-    float s = 0.0f;
-    for (int z = 0; z < splits; ++z) s += partial[(long long)z * M * N + i];
-    out[i] = s;
-}
+// One gradient reduction of a training step: the split sum of a
+// weight-gradient product (out[m][n] = sum over z, in order, of
+// partial[z][m][n]) or the column sums of a layer's dz [rows][N] (its bias
+// gradient).
+struct GradReduce {
+    const float* src;  // partial [splits][rows][N], or dz [rows][N]
+    float* out;
+    long long rows;
+    int N;
+    int splits;  // > 0: a split sum; 0: column sums
+};
+constexpr int kGradReduces = 8;  // the four weight and four bias gradients
+struct GradReduces {
+    GradReduce r[kGradReduces];
+};
 
-
-// Bias gradient: column sums of dz [rows][N].
-__global__ void colsum_kernel(const float* dz, long long rows, int N, float* out)
+// Every gradient reduction of a step in one launch: blockIdx.y picks the
+// reduction; a split sum takes 256 elements a block, column sums one column
+// a block (256 threads, partial sums per wave, then the four waves in order).
+__global__ void __launch_bounds__(256) grad_reduce_kernel(GradReduces rs)
 {
+    const GradReduce g = rs.r[blockIdx.y];
+    if (g.splits > 0) {
+        const long long mn = g.rows * g.N;
+        const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= mn) return;
+        float s = 0.0f;
+        for (int z = 0; z < g.splits; ++z) s += g.src[(long long)z * mn + i];
+        g.out[i] = s;
+        return;
+    }
     const int n = blockIdx.x;
+    if (n >= g.N) return;  // the whole block
     float s = 0.0f;
-    for (long long m = threadIdx.x; m < rows; m += blockDim.x) s += dz[m * N + n];
+    for (long long m = threadIdx.x; m < g.rows; m += blockDim.x) s += g.src[m * g.N + n];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     __shared__ float part[4];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[n] = part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x == 0) g.out[n] = part[0] + part[1] + part[2] + part[3];
 }
 
 // Input patches of a batch in the [feature][sample] orientation (for dW1).
@@ -406,9 +424,11 @@ hipError_t gemm(const ALoad& a, const __bf16* bt, long long M, int N, int K, con
 }
 
 // Weight gradient g[M][N] = A . Bt^T with the batch (K) split over
-// kGradSplits workgroups along z, the partials summed in a fixed order (the
-// result does not depend on scheduling).  A has M rows, Bt N rows, both K long.
-hipError_t gemm_weight_grad(const __bf16* a, const __bf16* bt, int M, int N, int K, float* partial, float* g,
+// kGradSplits workgroups along z: the partial products go to `partial`, and
+// the returned reduction (run with the step's others by grad_reduce_kernel)
+// sums them in a fixed order, so the result does not depend on scheduling.
+// A has M rows, Bt N rows, both K long.
+GradReduce gemm_weight_grad(const __bf16* a, const __bf16* bt, int M, int N, int K, float* partial, float* g,
                             hipStream_t s)
 {
     int splits = kGradSplits;
@@ -417,10 +437,7 @@ hipError_t gemm_weight_grad(const __bf16* a, const __bf16* bt, int M, int N, int
     const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)splits);
     hipLaunchKernelGGL((gemm_kernel<DenseRows, EpiPartial>), grid, dim3(256), 0, s, DenseRows{a, K}, DenseRows{bt, K},
                        (long long)M, N, ks, EpiPartial{partial, M, N});
-    const long long mn = (long long)M * N;
-    hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, partial, splits,
-                       (long long)M, N, g);
-    return hipGetLastError();
+    return GradReduce{partial, g, (long long)M, N, splits};
 }
 
 }  // namespace mlp
@@ -456,10 +473,13 @@ hipError_t mlp_step(const Model& md, const TrainBatch& tb, const TrainBuffers& b
     if ((e = gemm(DenseRows{buf.h[2], kHidden}, md.wt[3], B, kOutPad, kHidden,
                   EpiOutputTrain{md.w + kOffB[3], tb.targets, buf.idx, B, tb.outputs, tb.layers, tb.w, tb.h,
                                  buf.dz[3], buf.dzb[3], buf.dzt[3], loss}, s)) != hipSuccess) return e;
-    // backward: weight gradients, then the gradient flowing into the layer below
-    if ((e = gemm_weight_grad(buf.ht[2], buf.dzt[3], kHidden, kOutPad, (int)B, buf.partial, buf.grad + kOffW[3],
-                              s)) != hipSuccess)
-        return e;
+    // backward: weight-gradient products (each layer's partials in a slot of
+    // their own), the gradient flowing into the layer below, then every
+    // weight and bias gradient reduction in one launch
+    const size_t slot = size_t(kGradSplits) * kHidden * kHidden;
+    GradReduces red{};
+    red.r[3] = gemm_weight_grad(buf.ht[2], buf.dzt[3], kHidden, kOutPad, (int)B, buf.partial + 3 * slot,
+                                buf.grad + kOffW[3], s);
     for (int l = 2; l >= 0; --l) {
         const int nxt = l + 1;
         const int kin = (nxt == 3) ? kOutPad : kHidden;
@@ -467,21 +487,27 @@ hipError_t mlp_step(const Model& md, const TrainBatch& tb, const TrainBuffers& b
                       EpiBackHidden{buf.z[l], buf.dz[l], buf.dzb[l], buf.dzt[l], B, kHidden}, s)) != hipSuccess)
             return e;
         if (l > 0) {
-            if ((e = gemm_weight_grad(buf.ht[l - 1], buf.dzt[l], kHidden, kHidden, (int)B, buf.partial,
-                                      buf.grad + kOffW[l], s)) != hipSuccess)
-                return e;
+            red.r[l] = gemm_weight_grad(buf.ht[l - 1], buf.dzt[l], kHidden, kHidden, (int)B, buf.partial + l * slot,
+                                        buf.grad + kOffW[l], s);
         } else {
             hipLaunchKernelGGL(patch_transpose_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, pr, B,
                                buf.xt);
-            if ((e = gemm_weight_grad(buf.xt, buf.dzt[0], kInPad, kHidden, (int)B, buf.partial, buf.grad + kOffW[0],
-                                      s)) != hipSuccess)
-                return e;
+            red.r[0] = gemm_weight_grad(buf.xt, buf.dzt[0], kInPad, kHidden, (int)B, buf.partial, buf.grad + kOffW[0],
+                                        s);
         }
     }
     for (int l = 0; l < 4; ++l) {
         const int N = (l == 3) ? kOutPad : kHidden;
-        hipLaunchKernelGGL(colsum_kernel, dim3(N), dim3(256), 0, s, buf.dz[l], B, N, buf.grad + kOffB[l]);
+        red.r[4 + l] = GradReduce{buf.dz[l], buf.grad + kOffB[l], B, N, 0};
     }
+    // blocks along x: the largest split sum (128 x 128 / 256) or column count
+    unsigned bx = 1;
+    for (int k = 0; k < kGradReduces; ++k) {
+        const GradReduce& g = red.r[k];
+        const long long need = g.splits > 0 ? (g.rows * g.N + 255) / 256 : g.N;
+        bx = std::max<unsigned>(bx, (unsigned)need);
+    }
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(bx, kGradReduces), dim3(256), 0, s, red);
     if (update) {
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f - powf(b1, (float)(step + 1)), c2 = 1.0f - powf(b2, (float)(step + 1));

@@ -100,7 +100,7 @@ struct TrainBuffers {
     __bf16* xt;
     float* grad;
     float* loss;  // kLossSlots per-step losses
-    float* partial;  // split weight-gradient products [kGradSplits][128][128]
+    float* partial;  // split weight-gradient products, a slot each layer: [4][kGradSplits][128][128]
 };
 
 constexpr int kGradSplits = 32;  // workgroups along the batch of a weight-gradient product

@@ -369,7 +369,7 @@ mlp::TrainBuffers mlp_train_view(void* ws, int batch)
     }
     b.xt = carve<__bf16>(cur, B * kInPad);
     b.grad = carve<float>(cur, kParamsPadded);
-    b.partial = carve<float>(cur, size_t(mlp::kGradSplits) * kHidden * kHidden);
+    b.partial = carve<float>(cur, 4 * size_t(mlp::kGradSplits) * kHidden * kHidden);
     b.loss = carve<float>(cur, mlp::kLossSlots);
     return b;
 }
