@@ -251,14 +251,36 @@ struct GradReduce {
 constexpr int kGradReduces = 8;  // the four weight and four bias gradients
 struct GradReduces {
     GradReduce r[kGradReduces];
+    // Adam (Kingma & Ba) on each parameter as its gradient is formed, when
+    // update is set: the reductions' outputs cover every parameter once
+    Model md;
+    const float* grad;  // the gradient vector the outputs lie in
+    float lr, b1, b2, eps, c1, c2;
+    int update;
 };
 
-// Every gradient reduction of a step in one launch: blockIdx.y picks the
-// reduction; a split sum takes 256 elements a block, column sums one column
-// a block (256 threads, partial sums per wave, then the four waves in order).
+// Adam on parameter i with gradient g: the FP32 master weight, then both
+// BF16 copies.
+__device__ __forceinline__ void adam_param(const Model& md, int i, float g, float lr, float b1, float b2, float eps,
+                                           float c1, float c2)
+{
+    const float m = b1 * md.m[i] + (1.0f - b1) * g;
+    const float v = b2 * md.v[i] + (1.0f - b2) * g * g;
+    md.m[i] = m;
+    md.v[i] = v;
+    const float w = md.w[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    md.w[i] = w;
+    write_bf16(md, i, w);
+}
+
+// Every gradient reduction of a step in one launch, each gradient followed
+// by its parameter's Adam update: blockIdx.y picks the reduction; a split sum
+// takes 256 elements a block, column sums one column a block (256 threads,
+// partial sums per wave, then the four waves in order).
 __global__ void __launch_bounds__(256) grad_reduce_kernel(GradReduces rs)
 {
     const GradReduce g = rs.r[blockIdx.y];
+    const int base = int(g.out - rs.grad);  // the parameter index of out[0]
     if (g.splits > 0) {
         const long long mn = g.rows * g.N;
         const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,6 +288,7 @@ __global__ void __launch_bounds__(256) grad_reduce_kernel(GradReduces rs)
         float s = 0.0f;
         for (int z = 0; z < g.splits; ++z) s += g.src[(long long)z * mn + i];
         g.out[i] = s;
+        if (rs.update) adam_param(rs.md, base + int(i), s, rs.lr, rs.b1, rs.b2, rs.eps, rs.c1, rs.c2);
         return;
     }
     const int n = blockIdx.x;
@@ -276,7 +299,11 @@ __global__ void __launch_bounds__(256) grad_reduce_kernel(GradReduces rs)
     __shared__ float part[4];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) g.out[n] = part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x == 0) {
+        const float t = part[0] + part[1] + part[2] + part[3];
+        g.out[n] = t;
+        if (rs.update) adam_param(rs.md, base + n, t, rs.lr, rs.b1, rs.b2, rs.eps, rs.c1, rs.c2);
+    }
 }
 
 // Input patches of a batch in the [feature][sample] orientation (for dW1).
@@ -307,21 +334,6 @@ __global__ void sample_kernel(long long* idx, int batch, long long pool, unsigne
     if (b >= batch) return;
     const unsigned long long r = splitmix(seed ^ splitmix(((unsigned long long)step << 32) | (unsigned)b));
     idx[b] = (long long)(r % (unsigned long long)pool);
-}
-
-// Adam (Kingma & Ba) on the FP32 master weights, then both BF16 copies.
-__global__ void adam_kernel(Model md, const float* grad, float lr, float b1, float b2, float eps, float c1, float c2)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kParamsPadded) return;
-    const float g = grad[i];
-    const float m = b1 * md.m[i] + (1.0f - b1) * g;
-    const float v = b2 * md.v[i] + (1.0f - b2) * g * g;
-    md.m[i] = m;
-    md.v[i] = v;
-    const float w = md.w[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
-    md.w[i] = w;
-    write_bf16(md, i, w);
 }
 
 __global__ void refresh_bf16_kernel(Model md)
@@ -507,13 +519,16 @@ hipError_t mlp_step(const Model& md, const TrainBatch& tb, const TrainBuffers& b
         const long long need = g.splits > 0 ? (g.rows * g.N + 255) / 256 : g.N;
         bx = std::max<unsigned>(bx, (unsigned)need);
     }
+    red.md = md;
+    red.grad = buf.grad;
+    red.update = update ? 1 : 0;
+    red.b1 = 0.9f;
+    red.b2 = 0.999f;
+    red.eps = 1e-8f;
+    red.lr = lr;
+    red.c1 = 1.0f - powf(red.b1, (float)(step + 1));
+    red.c2 = 1.0f - powf(red.b2, (float)(step + 1));
     hipLaunchKernelGGL(grad_reduce_kernel, dim3(bx, kGradReduces), dim3(256), 0, s, red);
-    if (update) {
-        const float b1 = 0.9f, b2 = 0.999f;
-        const float c1 = 1.0f - powf(b1, (float)(step + 1)), c2 = 1.0f - powf(b2, (float)(step + 1));
-        hipLaunchKernelGGL(adam_kernel, dim3((kParamsPadded + 255) / 256), dim3(256), 0, s, md, buf.grad, lr, b1, b2,
-                           1e-8f, c1, c2);
-    }
     return hipGetLastError();
 }
 
