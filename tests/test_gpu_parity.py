@@ -184,6 +184,27 @@ def test_refine_first_pass_may_alias_output():
     assert not np.array_equal(separate, first)
 
 
+def test_host_calls_repeat_and_release_kept_buffers():
+    """The host-pointer calls keep their device buffers between calls: calls
+    of different sizes in turn (the kept buffers grown, then reused larger
+    than needed), and calls after unblending_hip_release_buffers, all give
+    the first results."""
+    import unblending_amd as U
+    work = [_workload(1, (40, 36)), _workload(2, (64, 60)), _workload(3, (20, 18))]
+    print("case kept host buffers", flush=True)
+    ref = []
+    for wl in work:
+        a = U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True)
+        ref.append((a, U.perform_matte_refinement(wl.rgb, a, layer_infos(wl.layers), True, False)))
+    for rep in range(2):
+        for wl, (a0, b0) in zip(work, ref):
+            a = U.compute_color_unmixing(wl.rgb, layer_infos(wl.layers), True)
+            assert np.array_equal(a, a0), (rep, wl.name)
+            assert np.array_equal(U.perform_matte_refinement(wl.rgb, a, layer_infos(wl.layers), True, False), b0)
+        U.release_device_buffers()
+    U.release_device_buffers()  # with nothing kept
+
+
 def test_concurrent_calls_from_two_threads():
     """The C ABI is re-entrant (each call has its own stream and buffers, and
     a call that finds the shared pinned staging buffers in use stages through

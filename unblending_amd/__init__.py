@@ -258,6 +258,8 @@ for _f in (_lib.unblending_hip_unmix_batch_device, _lib.unblending_hip_refine_ba
     _f.argtypes = [ctypes.POINTER(HipImage), ctypes.c_size_t, ctypes.POINTER(Options), ctypes.POINTER(HipParams),
                    _vp, ctypes.c_size_t, _vp, ctypes.POINTER(HipStats)]
 _lib.unblending_hip_last_error.restype = ctypes.c_char_p
+_lib.unblending_hip_release_buffers.restype = ctypes.c_int32
+_lib.unblending_hip_release_buffers.argtypes = []
 
 
 _lib.unblending_hip_composite_device.restype = ctypes.c_int32
@@ -280,6 +282,12 @@ def abi_version() -> int:
 
 def last_error() -> str:
     return _lib.unblending_hip_last_error().decode()
+
+
+def release_device_buffers():
+    """Frees the device buffers the host-pointer calls keep on the current
+    device between calls (unblending_hip_release_buffers)."""
+    raise_for_status(_lib.unblending_hip_release_buffers())
 
 
 def layer_array(layer_infos) -> ctypes.Array:

@@ -784,4 +784,16 @@ UNBLENDING_API int32_t unblending_hip_mlp_predict(const void* model, const doubl
 
 UNBLENDING_API const char* unblending_hip_last_error(void) { return g_last_error.c_str(); }
 
+UNBLENDING_API int32_t unblending_hip_release_buffers(void)
+{
+    g_last_error.clear();
+    try {
+        release_host_buffers();
+        return UNBLENDING_OK;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return UNBLENDING_DEVICE_ERROR;
+    }
+}
+
 }  // extern "C"

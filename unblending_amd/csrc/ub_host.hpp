@@ -79,6 +79,8 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
                unblending_hip_stats* stats);
 void host_solve(const unblending_rgb_view& rgb, const LayerSet& set, const double* first, bool opaque,
                 bool smooth, const unblending_hip_params& params, double* output, unblending_hip_stats* stats);
+// frees the device buffers host_solve keeps between calls on the current device
+void release_host_buffers();
 
 void run_composite(const double* layers, int w, int h, size_t count, const int32_t* comp_ops, const int32_t* modes,
                    double* out, hipStream_t stream);
