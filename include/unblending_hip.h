@@ -254,13 +254,13 @@ UNBLENDING_API int32_t uhip_solve_planes(
 UNBLENDING_API const char* unblending_hip_last_error(void);
 
 /* No counterpart in the reference.  The host-pointer entries
- * (unblending_unmix, unblending_refine and their _ex forms, uhip_solve_planes)
- * keep their device buffers -- image, first pass, result and workspace, the
- * largest up to ~31 GB for a 4096^2 7-layer image -- on each device between
- * calls, growing them when a call needs more, because allocating them again
- * at every call makes the driver clear the memory each time.  This frees the
- * buffers kept on the current device (after a call in progress on another
- * thread).  Returns UNBLENDING_OK. */
+ * (unblending_unmix, unblending_refine and their _ex forms, uhip_solve_planes,
+ * unblending_hip_composite) keep their device buffers -- image, first pass,
+ * result and workspace, the largest up to ~31 GB for a 4096^2 7-layer image --
+ * on each device between calls, growing them when a call needs more, because
+ * allocating them again at every call makes the driver clear the memory each
+ * time.  This frees the buffers kept on the current device (after a call in
+ * progress on another thread).  Returns UNBLENDING_OK. */
 UNBLENDING_API int32_t unblending_hip_release_buffers(void);
 
 #ifdef __cplusplus

@@ -875,15 +875,15 @@ void run_batch(const unblending_hip_image* images, const std::vector<LayerSet>& 
 
 namespace {
 
-// Device buffers of the host-pointer path (image, result, first pass,
-// workspace), kept between calls on each device and grown when a call needs
-// more.  Allocated and freed at every call they cost the driver's clearing
+// Device buffers of the host-pointer path (the solve's image, result, first
+// pass and workspace; the recomposite's layers and result), kept between
+// calls on each device and grown when a call needs more.  Allocated and freed at every call they cost the driver's clearing
 // of the memory at each allocation once its cleared memory is used up:
 // measured on an MI355X, a 1024^2, 5-layer unmix went from 70 ms to 620 ms
 // a call after its first calls (its workspace holds a 22 GB (s, y) history
 // region).  A call that finds them in use by another host thread allocates
 // its own; unblending_hip_release_buffers frees them.
-constexpr int kHostBufferKinds = 4;
+constexpr int kHostBufferKinds = 6;  // solve: image, result, first pass, workspace; composite: layers, result
 struct HostBuffers {
     std::mutex mutex;
     void* p[kHostBufferKinds] = {};
@@ -1029,20 +1029,24 @@ void host_composite(const double* layers, int w, int h, size_t count, const int3
     require_device();
     const size_t n = size_t(w) * size_t(h);
     Stream stream;
-    DeviceBuffer d_in(4 * count * n * sizeof(double));
-    DeviceBuffer d_out(4 * n * sizeof(double));
+    HostBuffers* hb = host_buffers();
+    std::unique_lock<std::mutex> held;
+    if (hb) held = std::unique_lock<std::mutex>(hb->mutex, std::try_to_lock);
+    const bool keep = held.owns_lock();
+    HostBuffer d_in(hb, keep, 4, 4 * count * n * sizeof(double));
+    HostBuffer d_out(hb, keep, 5, 4 * n * sizeof(double));
     with_staging([&](Staging* st) {
-        if (st) st->upload(d_in.p, layers, 4 * count * n * sizeof(double), stream.s);
+        if (st) st->upload(d_in.get(), layers, 4 * count * n * sizeof(double), stream.s);
         else
-            check(hipMemcpyAsync(d_in.p, layers, 4 * count * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
+            check(hipMemcpyAsync(d_in.get(), layers, 4 * count * n * sizeof(double), hipMemcpyHostToDevice, stream.s),
                   "upload layers");
     });
     run_composite(d_in.as<double>(), w, h, count, comp_ops, modes, d_out.as<double>(), stream.s);
     with_staging([&](Staging* st) {
         if (st) {
-            st->download(out, d_out.p, 4 * n * sizeof(double), stream.s);
+            st->download(out, d_out.get(), 4 * n * sizeof(double), stream.s);
         } else {
-            check(hipMemcpyAsync(out, d_out.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream.s), "download");
+            check(hipMemcpyAsync(out, d_out.get(), 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream.s), "download");
             check(hipStreamSynchronize(stream.s), "composite");
         }
     });
