@@ -549,18 +549,25 @@ def main():
     if not args.no_abi:
         rgb_host = np.ascontiguousarray(wl.rgb)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        a_first = U.compute_color_unmixing(rgb_host, infos, True, params=params)
-        t1 = time.perf_counter()
-        a_out = U.perform_matte_refinement(rgb_host, a_first, infos, True, False, params=params)
-        t2 = time.perf_counter()
-        abi = {"value": round(n / (t2 - t0) / 1e6, 4), "unit": "Mpixel/s", "unmix_s": round(t1 - t0, 3),
-               "refine_s": round(t2 - t1, 3),
+        calls = []
+        for _ in range(2):  # the first pair allocates the kept device buffers, the second reuses them
+            t0 = time.perf_counter()
+            a_first = U.compute_color_unmixing(rgb_host, infos, True, params=params)
+            t1 = time.perf_counter()
+            a_out = U.perform_matte_refinement(rgb_host, a_first, infos, True, False, params=params)
+            t2 = time.perf_counter()
+            calls.append((t1 - t0, t2 - t1))
+        (u1, r1), (u2, r2) = calls
+        abi = {"value": round(n / (u2 + r2) / 1e6, 4), "unit": "Mpixel/s", "unmix_s": round(u2, 3),
+               "refine_s": round(r2, 3), "first_call": {"value": round(n / (u1 + r1) / 1e6, 4),
+                                                        "unmix_s": round(u1, 3), "refine_s": round(r1, 3)},
                "bit_identical_to_device_path": bool(np.array_equal(a_first, first_plain.cpu().numpy()) and
                                                     np.array_equal(a_out, out_plain.cpu().numpy())),
-               "note": "unblending_unmix_ex + unblending_refine_ex on host buffers, one call each: device "
-                       "allocation, host-device copies and repacks included; not the value"}
-        log("host-pointer interface: %.2f s + %.2f s" % (t1 - t0, t2 - t1))
+               "note": "unblending_unmix_ex + unblending_refine_ex on host buffers, the second pair of calls "
+                       "(first_call: the first pair, which allocates the device buffers the calls keep): "
+                       "host-device copies and repacks included; not the value"}
+        log("host-pointer interface: %.2f s + %.2f s, then %.2f s + %.2f s" % (u1, r1, u2, r2))
+        U.release_device_buffers()
         del a_first, a_out
 
     # ---- the one-pair fast mode, against the reported step's outputs ----
