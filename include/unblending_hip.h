@@ -193,7 +193,12 @@ UNBLENDING_API int32_t unblending_hip_mlp_predict(
  * the per-pixel results are those of unblending_hip_unmix_device /
  * unblending_hip_refine_device on each image alone.  The options apply to
  * every image.  Enqueued on `stream`; no allocation, no synchronisation
- * unless stats are asked for, so a call can be captured in a HIP graph.
+ * unless stats are asked for, so a call can be captured in a HIP graph.  The
+ * refinement filters up to four images at once on streams of the library's
+ * own, forked from and joined to `stream` by events (a graph capture records
+ * them as branches); they are made at the first batch refinement on a device
+ * outside a capture and kept (a refinement captured before then filters on
+ * `stream` alone).
  * With params.warm_start = 1, the first pass starts each image that has a
  * model (unblending_hip_image.mlp) from that model's prediction, as
  * unblending_hip_unmix_device does for one image (params.mlp is not used). */
